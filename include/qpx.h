@@ -1,0 +1,114 @@
+/*
+ * qpx.h -- C-ABI of libqpx, the MI355X-native determinant-driven CI hot path
+ * (Davidson sigma H.Psi / S^2.Psi and CIPSI selection with Epstein-Nesbet
+ * PT2) of Quantum Package v1.
+ *
+ * Plain C types only.  Every entry point returns 0 on success and a nonzero
+ * code on failure; qpx_last_error() gives the message.  One context drives
+ * one GPU; call it from one host thread.  Host arrays are owned by the
+ * caller and only read unless documented; device memory is owned by the
+ * context.
+ *
+ * Conventions (as the reference's providers):
+ *   - determinants are det(N_int,2) 64-bit words, alpha then beta; orbital p
+ *     (1-based) is bit (p-1)%64 of word (p-1)/64
+ *     (src/Bitmask/bitmasks_module.f90:1-11);
+ *   - matrices are column-major (Fortran);
+ *   - two-electron integrals are 8-fold packed: <ij|kl> = (ik|jl) at index
+ *     key-1, key = tri(tri(i,k),tri(j,l)), tri(a,b) = min+max(max-1)/2
+ *     (src/Integrals_Bielec/map_integrals.irp.f:19-34).
+ */
+#ifndef QPX_H
+#define QPX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qpx_ctx qpx_ctx;
+
+enum { QPX_FP64 = 0, QPX_FP32 = 1 };
+
+/* Create a context on HIP device `device`.
+ * No reference counterpart: replaces the implicit provider state. */
+int qpx_create(int device, qpx_ctx **ctx);
+int qpx_destroy(qpx_ctx *ctx);
+/* Copies the last error message (NUL-terminated) into buf. */
+int qpx_last_error(qpx_ctx *ctx, char *buf, int len);
+
+/* Hamiltonian: h_core (M x M, column-major) = mo_mono_elec_integral
+ * (src/Integrals_Monoelec/mo_mono_ints.irp.f:1-25); eri_packed (n8) holds
+ * the MO integrals keyed as mo_bielec_integral
+ * (src/Integrals_Bielec/map_integrals.irp.f:383-428).  Derives the
+ * device-side Coulomb/exchange tables (mo_bi_integrals.irp.f:1344-1369).
+ * Invalidates the wavefunction.  */
+int qpx_set_hamiltonian(qpx_ctx *ctx, int mo_num, int n_alpha, int n_beta, const double *h_core,
+                        const double *eri_packed, int64_t n8);
+
+/* Wavefunction determinants psi_det (N_int,2,n_det).  Builds the unique
+ * spin strings and the alpha x beta bilinear structure
+ * (src/Determinants/spindeterminants.irp.f:56-136,387-597) plus the
+ * spin-string connectivity used by sigma.  Call again after psi changes
+ * (SOFT_TOUCH psi_det, src/Determinants/H_apply.irp.f:193). */
+int qpx_set_wavefunction(qpx_ctx *ctx, int n_int, int64_t n_det, const uint64_t *psi_det);
+
+/* H_S2_u_0_nstates_openmp (src/Davidson/u0Hu0.irp.f:15-66):
+ * v = H u, s = S^2 u for n_st vectors.  u, v, s are (sze, n_st)
+ * column-major in psi_det order, sze >= n_det.  u is not modified (the
+ * reference permutes it in place and restores it).  prec = QPX_FP64 or
+ * QPX_FP32 (FP32 u, integrals and accumulation; results returned as
+ * double). */
+int qpx_sigma(qpx_ctx *ctx, int prec, int n_st, int64_t sze, const double *u, double *v, double *s);
+
+/* H_S2_u_0_nstates_openmp_work (src/Davidson/u0Hu0.irp.f:69-96) on device
+ * memory: u_t, v_t, s_t are (n_st, n_det) state-fastest in the context's
+ * bilinear (beta-column-major) order, of type double (QPX_FP64) or float
+ * (QPX_FP32).  v_t and s_t are overwritten.  stream may be NULL (the
+ * context's stream). */
+int qpx_sigma_device(qpx_ctx *ctx, int prec, int n_st, const void *u_t, void *v_t, void *s_t, void *stream);
+
+/* Permutation of the context's bilinear order: perm[k] = psi_det index
+ * (0-based) of bilinear position k (psi_bilinear_matrix_order - 1). */
+int qpx_get_bilinear_order(qpx_ctx *ctx, int64_t *perm);
+
+/* Diagonal <I|H|I> for every determinant, psi_det order
+ * (diag_H_mat_elem, src/Determinants/slater_rules.irp.f:1869-1929). */
+int qpx_diag_h(qpx_ctx *ctx, double *h_jj);
+
+/* davidson_diag_hs2 (src/Davidson/diagonalization_hs2.irp.f:1-444).
+ * u_in (dim_in, n_st_diag) column-major: guess vectors on entry (columns
+ * n_st..n_st_diag-1 are replaced by seeded random vectors as the reference
+ * does with random_number), eigenvectors on exit.  energies and s2_out have
+ * n_st_diag entries.  mixed = 0: FP64 sigma throughout; mixed = 1: FP32
+ * sigma for the correction vectors until the squared residual falls below
+ * switch_threshold, then FP64 sigma (Rayleigh quotient, orthogonalisation,
+ * residuals always FP64).  n_sigma_out (may be NULL) receives
+ * {number of FP64 sigma calls, number of FP32 sigma calls}. */
+int qpx_davidson(qpx_ctx *ctx, int n_st, int n_st_diag, double threshold, int sze_max, int s2_eig,
+                 int state_following, int mixed, double switch_threshold, uint64_t seed, double *u_in,
+                 int64_t dim_in, double *energies, double *s2_out, int *n_sigma_out);
+
+/* ZMQ_selection (plugins/Full_CI_ZMQ/zmq_selection.irp.f:1-100) over the
+ * current wavefunction.  psi must have been set in psi_det_sorted order
+ * (decreasing state-averaged weight, src/Determinants/determinants.irp.f:258-293);
+ * coef (n_det, n_states) column-major are psi_coef_sorted.  The first
+ * n_gen determinants are the generators, the first n_sel the selectors.
+ * E0 (n_states) is pt2_E0_denominator.  On exit pt2 (n_states) holds the
+ * EN-PT2 sum and out_dets (N_int,2,n_keep) / out_vals (n_keep) the n_keep
+ * external determinants with the most negative min-state contribution
+ * (ascending values), *n_out their count.  subset = 0 processes every
+ * block; subset > 0 keeps the blocks with maskInd mod fragment_count ==
+ * subset-1 (selection.irp.f:561-562). */
+int qpx_select(qpx_ctx *ctx, int n_states, const double *coef, int n_gen, int n_sel, const double *E0,
+               int64_t n_keep, int subset, double *pt2, uint64_t *out_dets, double *out_vals, int64_t *n_out);
+
+/* Counters of the last qpx_select call: [0] candidates evaluated (unbanned,
+ * nonzero coupling), [1] blocks processed, [2] blocks visited, [3] integral
+ * and coefficient gathers. */
+int qpx_select_stats(qpx_ctx *ctx, int64_t *stats4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

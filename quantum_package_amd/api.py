@@ -1,0 +1,161 @@
+"""Host-side mirror of the reference's provider/subroutine interface for the
+determinant-driven CI hot path, backed by libqpx on the GPU.
+
+Names follow the reference:
+  H_S2_u_0_nstates   -> H_S2_u_0_nstates_openmp   (src/Davidson/u0Hu0.irp.f:15)
+  u_0_H_u_0          -> u_0_H_u_0                 (src/Davidson/diagonalization_hs2.irp.f:446)
+  davidson_diag_hs2  -> davidson_diag_hs2         (src/Davidson/diagonalization_hs2.irp.f:1)
+  ZMQ_selection      -> ZMQ_selection             (plugins/Full_CI_ZMQ/zmq_selection.irp.f:1)
+Arrays follow the reference layouts: psi_det (N_det, 2, N_int) uint64 (i.e.
+Fortran det(N_int,2,N_det)), vectors (sze, N_st) column-major.  Errors raise
+QpxError, the counterpart of the reference's ``stop`` on fatal errors.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+FP64, FP32 = 0, 1
+
+
+class QpxError(RuntimeError):
+    pass
+
+
+def _dp(a):
+    return a.ctypes.data_as(_lib.c_dp)
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.L = _lib.load()
+        self.ptr = C.c_void_p()
+        rc = self.L.qpx_create(device, C.byref(self.ptr))
+        if rc != 0:
+            raise QpxError(f"qpx_create failed (code {rc}): no usable HIP device {device}")
+        self.H = None
+        self.n_det = 0
+        self.n_int = 0
+
+    def _check(self, rc):
+        if rc != 0:
+            buf = C.create_string_buffer(1024)
+            self.L.qpx_last_error(self.ptr, buf, 1024)
+            raise QpxError(buf.value.decode(errors="replace") or f"libqpx error {rc}")
+
+    def close(self):
+        if self.ptr:
+            self.L.qpx_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- inputs
+    def set_hamiltonian(self, H):
+        h = np.asfortranarray(H.h, dtype=np.float64)
+        eri = np.ascontiguousarray(H.eri, dtype=np.float64)
+        self._check(self.L.qpx_set_hamiltonian(self.ptr, H.M, H.n_alpha, H.n_beta, _dp(h), _dp(eri), eri.size))
+        self.H = H
+
+    def set_wavefunction(self, psi_det):
+        d = np.ascontiguousarray(psi_det, dtype=np.uint64)
+        if d.ndim != 3 or d.shape[1] != 2:
+            raise QpxError("psi_det must have shape (N_det, 2, N_int)")
+        self._check(self.L.qpx_set_wavefunction(self.ptr, d.shape[2], d.shape[0], d.ctypes.data_as(_lib.c_u64p)))
+        self.n_det = d.shape[0]
+        self.n_int = d.shape[2]
+
+    def bilinear_order(self) -> np.ndarray:
+        perm = np.zeros(self.n_det, dtype=np.int64)
+        self._check(self.L.qpx_get_bilinear_order(self.ptr, perm.ctypes.data_as(_lib.c_i64p)))
+        return perm
+
+    def diag_H(self) -> np.ndarray:
+        out = np.zeros(self.n_det)
+        self._check(self.L.qpx_diag_h(self.ptr, _dp(out)))
+        return out
+
+    # ----------------------------------------------------------------- sigma
+    def H_S2_u_0_nstates(self, u_0, prec: int = FP64):
+        """v_0 = H u_0, s_0 = S^2 u_0; u_0 (sze, N_st) or (sze,)."""
+        u = np.asfortranarray(u_0, dtype=np.float64)
+        squeeze = u.ndim == 1
+        if squeeze:
+            u = u[:, None].copy(order="F")
+        sze, nst = u.shape
+        v = np.zeros((sze, nst), order="F")
+        s = np.zeros((sze, nst), order="F")
+        self._check(self.L.qpx_sigma(self.ptr, prec, nst, sze, _dp(u), _dp(v), _dp(s)))
+        if squeeze:
+            return v[:, 0], s[:, 0]
+        return v, s
+
+    def u_0_H_u_0(self, u_0, prec: int = FP64):
+        """e_0 = <u_0|H|u_0> per state (no division by the norm when it is
+        nonzero, as the reference)."""
+        u = np.asfortranarray(u_0, dtype=np.float64)
+        if u.ndim == 1:
+            u = u[:, None]
+        v, _ = self.H_S2_u_0_nstates(u, prec)
+        e = np.zeros(u.shape[1])
+        for i in range(u.shape[1]):
+            if np.dot(u[:, i], u[:, i]) != 0.0:
+                e[i] = np.dot(v[:, i], u[:, i])
+        return e
+
+    # -------------------------------------------------------------- Davidson
+    def davidson_diag_hs2(self, u_in, N_st: int, N_st_diag: int, threshold: float = 1e-10, sze_max: int = 8,
+                          s2_eig: bool = True, state_following: bool = False, mixed: bool = False,
+                          switch_threshold: float = 1e-7, seed: int = 0x5EED):
+        """Returns (energies[N_st_diag], s2[N_st_diag], u[sze, N_st_diag], n_sigma(fp64, fp32))."""
+        g = np.asarray(u_in, dtype=np.float64)
+        if g.ndim == 1:
+            g = g[:, None]
+        sze = g.shape[0]
+        u = np.zeros((sze, N_st_diag), order="F")
+        u[:, : min(g.shape[1], N_st_diag)] = g[:, :N_st_diag]
+        e = np.zeros(N_st_diag)
+        s2 = np.zeros(N_st_diag)
+        ns = (C.c_int * 2)()
+        rc = self.L.qpx_davidson(self.ptr, N_st, N_st_diag, threshold, sze_max, int(s2_eig), int(state_following),
+                                 int(mixed), switch_threshold, seed, _dp(u), sze, _dp(e), _dp(s2),
+                                 C.cast(ns, _lib.c_ip))
+        self._check(rc)
+        return e, s2, u, (ns[0], ns[1])
+
+    # ------------------------------------------------------------- selection
+    def ZMQ_selection(self, coef_sorted, n_gen: int, n_sel: int, E0, N: int, subset: int = 0):
+        """Selection + EN-PT2 over the current wavefunction (set in
+        psi_det_sorted order).  Returns (pt2[N_states], dets[n, 2, N_int],
+        vals[n])."""
+        coef = np.asfortranarray(coef_sorted, dtype=np.float64)
+        if coef.ndim == 1:
+            coef = coef[:, None].copy(order="F")
+        ns = coef.shape[1]
+        E0 = np.ascontiguousarray(np.atleast_1d(E0), dtype=np.float64)
+        pt2 = np.zeros(ns)
+        out_d = np.zeros((max(N, 1), 2, self.n_int), dtype=np.uint64)
+        out_v = np.zeros(max(N, 1))
+        n_out = C.c_int64(0)
+        self._check(self.L.qpx_select(self.ptr, ns, _dp(coef), n_gen, n_sel, _dp(E0), N, subset, _dp(pt2),
+                                      out_d.ctypes.data_as(_lib.c_u64p), _dp(out_v), C.byref(n_out)))
+        n = n_out.value
+        return pt2, out_d[:n].copy(), out_v[:n].copy()
+
+    def select_stats(self):
+        st = np.zeros(4, dtype=np.int64)
+        self._check(self.L.qpx_select_stats(self.ptr, st.ctypes.data_as(_lib.c_i64p)))
+        return dict(candidates=int(st[0]), blocks=int(st[1]), blocks_visited=int(st[2]), gathers=int(st[3]))
+
+    # ----------------------------------------------------------------- tests
+    def _force_sigma_path(self, path: int) -> int:
+        last = C.c_int(0)
+        self._check(self.L.qpx_debug_force_sigma_path(self.ptr, path, C.byref(last)))
+        return last.value

@@ -1,0 +1,1464 @@
+// selection.hip -- CIPSI selection with Epstein-Nesbet PT2 on the GPU.
+//
+// Reference: plugins/Full_CI_ZMQ/selection.irp.f (select_connected :50-79,
+// select_singles_and_doubles :258-578, fill_buffer_double :582-644,
+// splash_pq :647-735, get_d2/d1/d0 :738-1085, past_d1/d2 :1088-1126,
+// spot_isinwf :1130-1171), selection_buffer.irp.f:2-138 and
+// zmq_selection.irp.f:1-164.
+//
+// Work decomposition (one GPU replaces the ZeroMQ task farm):
+//   * per batch of generators, a near-list kernel scans psi once per
+//     generator and keeps, in index order, the selectors within 4
+//     excitations and the non-selectors within 2 (the reference's
+//     preinteresting / prefullinteresting lists);
+//   * one workgroup owns one (generator, h1, h2) block -- the reference's
+//     maskInd unit.  The block's coupling matrix mat(N_states, M, M) lives
+//     in LDS; cell (i,j) is owned by thread (i+j) mod 256, so every line of
+//     cells touched by get_d1 maps to distinct threads and each cell
+//     receives its terms in selector order, the same order as the
+//     reference (bit-identical FP64 sums);
+//   * bans (spot_isinwf, past_d1/d2, symmetrisation) are order-independent
+//     sets and are built in parallel with LDS atomics before any coupling is
+//     accumulated;
+//   * candidates with min-state e <= tau go to a global pool; tau tightens
+//     between batches to the n_keep-th best key (the reference's buffer
+//     threshold `mini`).  Blocks that hit a full pool are re-run after the
+//     pool is compacted.  Per-block PT2 is written to its own slot and summed
+//     in block order on the host.
+// Floating-point expressions keep the reference's operation order and this
+// file is compiled without FMA contraction.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+#include "qpx_internal.h"
+#include "qpx_device.h"
+
+using namespace qpx;
+
+namespace {
+
+constexpr int SB = 256;  // threads per block workgroup
+constexpr int MAXNS = 4;
+
+// ---------------------------------------------------------------- helpers
+template <int NI>
+struct Det {
+  uint64_t w[2 * NI];
+};
+
+__device__ inline bool bit(const uint64_t *s, int o) {  // o 1-based
+  return (s[(o - 1) >> 6] >> ((o - 1) & 63)) & 1;
+}
+
+// parity of occupied orbitals with index <= o in string s (get_mask_phase)
+template <int NI>
+__device__ inline int pm(const uint64_t *s, int o) {
+  int c = 0;
+#pragma unroll
+  for (int w = 0; w < NI; ++w) {
+    int lo = 64 * w;
+    if (o - 1 >= lo + 63) c += __popcll(s[w]);
+    else if (o - 1 >= lo) c += __popcll(s[w] & ((2ULL << ((o - 1) - lo)) - 1));
+  }
+  return c & 1;
+}
+
+template <int NI>
+__device__ inline double phase_bi(const uint64_t *sel, int s1, int s2, int h1, int p1, int h2, int p2) {
+  const uint64_t *a = sel + (s1 - 1) * NI, *b = sel + (s2 - 1) * NI;
+  int np = pm<NI>(a, h1) + pm<NI>(a, p1) + pm<NI>(b, h2) + pm<NI>(b, p2);
+  if (p1 < h1) np++;
+  if (p2 < h2) np++;
+  if (s1 == s2 && max(h1, p1) > min(h2, p2)) np++;
+  return (np & 1) ? -1.0 : 1.0;
+}
+
+struct Tabs {
+  const double *V, *hcore, *J, *JA, *fock_cs;
+  int M, npair;
+  double ref_energy;
+};
+
+__device__ inline double I4(const Tabs &T, int i, int j, int k, int l) {  // <ij|kl>, 1-based
+  return T.V[pair_index(i - 1, k - 1) * T.npair + pair_index(j - 1, l - 1)];
+}
+
+template <int NI>
+__device__ inline int count_between1(const uint64_t *a, int h, int p) {  // 1-based, strictly between
+  int lo = min(h, p), hi = max(h, p);
+  int c = 0;
+#pragma unroll
+  for (int w = 0; w < NI; ++w) {
+    int s = lo - 64 * w, e = hi - 1 - 64 * w;  // 0-based bit positions lo..hi-2 <=> orbitals lo+1..hi-1
+    if (s < 0) s = 0;
+    if (e > 64) e = 64;
+    if (s < e) {
+      uint64_t m = ((e == 64) ? ~0ULL : ((1ULL << e) - 1)) & ~((1ULL << s) - 1);
+      c += __popcll(a[w] & m);
+    }
+  }
+  return c;
+}
+
+// holes/particles of one spin between strings a (initial) and b (final),
+// 1-based, ascending
+template <int NI>
+__device__ inline void spin_lists(const uint64_t *a, const uint64_t *b, int *nh, int *hl, int *np, int *pl) {
+  *nh = 0;
+  *np = 0;
+#pragma unroll
+  for (int w = 0; w < NI; ++w) {
+    uint64_t x = a[w] ^ b[w];
+    uint64_t pp = x & b[w], hh = x & a[w];
+    while (pp) {
+      int o = 64 * w + __ffsll((unsigned long long)pp);
+      pp &= pp - 1;
+      if (*np < 2) pl[*np] = o;
+      (*np)++;
+    }
+    while (hh) {
+      int o = 64 * w + __ffsll((unsigned long long)hh);
+      hh &= hh - 1;
+      if (*nh < 2) hl[*nh] = o;
+      (*nh)++;
+    }
+  }
+}
+
+template <int NI>
+__device__ inline int spin_nperm(const uint64_t *a, int nh, const int *hl, const int *pl) {
+  int n = 0;
+  for (int l = 0; l < nh; ++l) n += count_between1<NI>(a, hl[l], pl[l]);
+  if (nh == 2) {
+    int A = min(hl[0], pl[0]), B = max(hl[0], pl[0]), Cc = min(hl[1], pl[1]), D = max(hl[1], pl[1]);
+    if (A < Cc && Cc < B && B < D) n++;
+  }
+  return n;
+}
+
+// diag_H_mat_elem (slater_rules.irp.f:1869-2022): incremental from the HF
+// reference with a/ac operators, in the reference's order
+template <int NI>
+__device__ double diag_H_incremental(const Tabs &T, const uint64_t *det, int na_e, int nb_e) {
+  const int M = T.M;
+  uint64_t ref[2 * NI];
+#pragma unroll
+  for (int w = 0; w < 2 * NI; ++w) ref[w] = 0;
+  for (int o = 1; o <= na_e; ++o) ref[(o - 1) >> 6] |= 1ULL << ((o - 1) & 63);
+  for (int o = 1; o <= nb_e; ++o) ref[NI + ((o - 1) >> 6)] |= 1ULL << ((o - 1) & 63);
+  uint64_t part[2 * NI], hole[2 * NI];
+  int nexc[2] = {0, 0};
+#pragma unroll
+  for (int w = 0; w < 2 * NI; ++w) {
+    uint64_t x = det[w] ^ ref[w];
+    part[w] = x & det[w];
+    hole[w] = x & ref[w];
+    nexc[w / NI] += __popcll(hole[w]);
+  }
+  double e = T.ref_energy;
+  if (nexc[0] + nexc[1] == 0) return e;
+  uint64_t key[2 * NI];
+#pragma unroll
+  for (int w = 0; w < 2 * NI; ++w) key[w] = ref[w];
+  int elec[2] = {na_e, nb_e};
+  for (int s = 0; s < 2; ++s) {
+    int na = elec[s], nb = elec[1 - s];
+    // i-th particle / hole of this spin, ascending
+    uint64_t pw[NI], hw[NI];
+#pragma unroll
+    for (int w = 0; w < NI; ++w) { pw[w] = part[s * NI + w]; hw[w] = hole[s * NI + w]; }
+    for (int i = 0; i < nexc[s]; ++i) {
+      int po = 0, ho = 0;
+#pragma unroll
+      for (int w = 0; w < NI; ++w)
+        if (!po && pw[w]) { po = 64 * w + __ffsll((unsigned long long)pw[w]); pw[w] &= pw[w] - 1; }
+#pragma unroll
+      for (int w = 0; w < NI; ++w)
+        if (!ho && hw[w]) { ho = 64 * w + __ffsll((unsigned long long)hw[w]); hw[w] &= hw[w] - 1; }
+      // ac_operator(po): loops over the current occupations before adding po
+      {
+        e += T.hcore[(po - 1) * (M + 1)];
+        int cnt = 0;
+        for (int w = 0; w < NI && cnt < na; ++w) {
+          uint64_t y = key[s * NI + w];
+          while (y && cnt < na) {
+            int o = 64 * w + __ffsll((unsigned long long)y);
+            y &= y - 1;
+            e += T.JA[(o - 1) + M * (po - 1)];
+            ++cnt;
+          }
+        }
+        cnt = 0;
+        for (int w = 0; w < NI && cnt < nb; ++w) {
+          uint64_t y = key[(1 - s) * NI + w];
+          while (y && cnt < nb) {
+            int o = 64 * w + __ffsll((unsigned long long)y);
+            y &= y - 1;
+            e += T.J[(o - 1) + M * (po - 1)];
+            ++cnt;
+          }
+        }
+        key[s * NI + ((po - 1) >> 6)] |= 1ULL << ((po - 1) & 63);
+        na += 1;
+      }
+      // a_operator(ho): removes ho, then loops over the remaining occupations
+      {
+        key[s * NI + ((ho - 1) >> 6)] &= ~(1ULL << ((ho - 1) & 63));
+        na -= 1;
+        e -= T.hcore[(ho - 1) * (M + 1)];
+        int cnt = 0;
+        for (int w = 0; w < NI && cnt < na; ++w) {
+          uint64_t y = key[s * NI + w];
+          while (y && cnt < na) {
+            int o = 64 * w + __ffsll((unsigned long long)y);
+            y &= y - 1;
+            e -= T.JA[(o - 1) + M * (ho - 1)];
+            ++cnt;
+          }
+        }
+        cnt = 0;
+        for (int w = 0; w < NI && cnt < nb; ++w) {
+          uint64_t y = key[(1 - s) * NI + w];
+          while (y && cnt < nb) {
+            int o = 64 * w + __ffsll((unsigned long long)y);
+            y &= y - 1;
+            e -= T.J[(o - 1) + M * (ho - 1)];
+            ++cnt;
+          }
+        }
+      }
+    }
+  }
+  return e;
+}
+
+// i_H_j (slater_rules.irp.f:478-571) with the Fock-based single excitation
+// (mono_excitations.irp.f:97-155); na_e/nb_e electron numbers
+template <int NI>
+__device__ double i_H_j(const Tabs &T, const uint64_t *ki, const uint64_t *kj, int na_e, int nb_e) {
+  int degree = 0;
+#pragma unroll
+  for (int w = 0; w < 2 * NI; ++w) degree += __popcll(ki[w] ^ kj[w]);
+  degree >>= 1;
+  if (degree > 2) return 0.0;
+  int nh[2], np[2], hl[2][2], pl[2][2];
+  spin_lists<NI>(ki, kj, &nh[0], hl[0], &np[0], pl[0]);
+  spin_lists<NI>(ki + NI, kj + NI, &nh[1], hl[1], &np[1], pl[1]);
+  if (degree == 2) {
+    int nperm = 0;
+    for (int s = 0; s < 2; ++s) {
+      if (nh[s] == 0) continue;
+      nperm += spin_nperm<NI>(ki + s * NI, nh[s], hl[s], pl[s]);
+      if (nh[s] == 2) break;
+    }
+    double phase = (nperm & 1) ? -1.0 : 1.0;
+    if (nh[0] == 1) {
+      return phase * I4(T, hl[0][0], hl[1][0], pl[0][0], pl[1][0]);
+    }
+    int s = (nh[0] == 2) ? 0 : 1;
+    return phase * (I4(T, hl[s][0], hl[s][1], pl[s][0], pl[s][1]) - I4(T, hl[s][0], hl[s][1], pl[s][1], pl[s][0]));
+  }
+  if (degree == 1) {
+    int s = (nh[0] == 1) ? 0 : 1;
+    double phase = (spin_nperm<NI>(ki + s * NI, 1, hl[s], pl[s]) & 1) ? -1.0 : 1.0;
+    // get_mono_excitation_from_fock(ki, kj, h:=particle, p:=hole, spin)
+    const int h = pl[s][0], p = hl[s][0];
+    const int M = T.M;
+    const int64_t Php = pair_index(h - 1, p - 1);
+    double hij = T.fock_cs[(h - 1) + M * (p - 1)];
+    // reference closed shell: orbitals 1..nb_e in both spins
+    uint64_t refcs[NI];
+#pragma unroll
+    for (int w = 0; w < NI; ++w) refcs[w] = 0;
+    for (int o = 1; o <= nb_e; ++o) refcs[(o - 1) >> 6] |= 1ULL << ((o - 1) & 63);
+    auto coul = [&](int i) { return T.V[Php * T.npair + pair_index(i - 1, i - 1)]; };
+    auto exch = [&](int i) { return T.V[pair_index(h - 1, i - 1) * T.npair + pair_index(i - 1, p - 1)]; };
+    for (int ss = 0; ss < 2; ++ss)
+#pragma unroll
+      for (int w = 0; w < NI; ++w) {
+        uint64_t y = (ki[ss * NI + w] ^ refcs[w]) & refcs[w];
+        while (y) { int o = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; hij -= coul(o); }
+      }
+#pragma unroll
+    for (int w = 0; w < NI; ++w) {
+      uint64_t y = (ki[s * NI + w] ^ refcs[w]) & refcs[w];
+      while (y) { int o = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; hij += exch(o); }
+    }
+    for (int ss = 0; ss < 2; ++ss)
+#pragma unroll
+      for (int w = 0; w < NI; ++w) {
+        uint64_t y = (ki[ss * NI + w] ^ refcs[w]) & ki[ss * NI + w];
+        while (y) { int o = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; hij += coul(o); }
+      }
+#pragma unroll
+    for (int w = 0; w < NI; ++w) {
+      uint64_t y = (ki[s * NI + w] ^ refcs[w]) & ki[s * NI + w];
+      while (y) { int o = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; hij -= exch(o); }
+    }
+    return hij * phase;
+  }
+  return diag_H_incremental<NI>(T, ki, na_e, nb_e);
+}
+
+// ---------------------------------------------------------------- near lists
+struct NearArgs {
+  const uint64_t *psi;
+  int64_t n_det;
+  int n_sel;
+  const int *gens;  // generator indices (1-based) of this batch
+  int n_gens;
+  const int64_t *off;  // fill offsets (per generator)
+  int *cnt;            // counts (per generator)
+  int *list;           // output det indices (1-based)
+};
+
+template <int NI>
+__global__ __launch_bounds__(256) void near_kernel(NearArgs A, int fill) {
+  __shared__ int wcount[4];
+  __shared__ int base;
+  const int g = A.gens[blockIdx.x];
+  const uint64_t *gen = A.psi + (int64_t)(g - 1) * 2 * NI;
+  uint64_t neg[2 * NI];
+#pragma unroll
+  for (int w = 0; w < 2 * NI; ++w) neg[w] = ~gen[w];
+  if (threadIdx.x == 0) base = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int64_t out0 = fill ? A.off[blockIdx.x] : 0;
+  for (int64_t t0 = 0; t0 < A.n_det; t0 += 256) {
+    int64_t i = t0 + threadIdx.x;
+    bool keep = false;
+    if (i < A.n_det) {
+      const uint64_t *d = A.psi + i * 2 * NI;
+      int nt = 0;
+#pragma unroll
+      for (int w = 0; w < 2 * NI; ++w) nt += __popcll(neg[w] & d[w]);
+      keep = (i + 1 <= A.n_sel) ? (nt <= 4) : (nt <= 2);
+    }
+    uint64_t m = __ballot(keep);
+    int my = __popcll(m & ((1ULL << lane) - 1));
+    if (lane == 0) wcount[wv] = __popcll(m);
+    __syncthreads();
+    int pre = 0, tot = 0;
+    for (int x = 0; x < 4; ++x) {
+      if (x < wv) pre += wcount[x];
+      tot += wcount[x];
+    }
+    if (fill && keep) A.list[out0 + base + pre + my] = (int)(i + 1);
+    __syncthreads();
+    if (threadIdx.x == 0) base += tot;
+    __syncthreads();
+  }
+  if (!fill && threadIdx.x == 0) A.cnt[blockIdx.x] = base;
+}
+
+// ---------------------------------------------------------------- blocks
+struct BlockDesc {
+  int gen;      // 1-based generator index
+  int gslot;    // index of the generator inside the batch (near-list slot)
+  int8_t s1, s2, unbanA, unbanB;
+  int16_t h1, h2;
+  int mask_ind;
+};
+
+struct SelArgs {
+  const uint64_t *psi;
+  const double *coef;  // (n_det, NS) column-major
+  int64_t n_det;
+  int n_sel, NS, na, nb;
+  double E0[MAXNS];
+  const BlockDesc *blocks;
+  int n_blocks;
+  const int64_t *near_off;
+  const int *near_cnt;
+  const int *near_list;
+  // pool
+  double *pool_key;
+  uint64_t *pool_det;
+  int *pool_blk;
+  unsigned long long *pool_count;
+  int64_t pool_cap;
+  double tau;
+  int blk_base;   // global id of blocks[0]
+  // outputs per block
+  double *blk_pt2;     // n_blocks * NS
+  int *blk_status;     // 0 done, 1 skipped (owned by an earlier generator), 2 overflow
+  int64_t *blk_ncand;
+  int64_t *blk_ngath;
+};
+
+template <int NI>
+__device__ inline void build_mask(const uint64_t *gen, int s1, int h1, int s2, int h2, uint64_t *mask) {
+#pragma unroll
+  for (int w = 0; w < 2 * NI; ++w) mask[w] = gen[w];
+  mask[(s1 - 1) * NI + ((h1 - 1) >> 6)] &= ~(1ULL << ((h1 - 1) & 63));
+  mask[(s2 - 1) * NI + ((h2 - 1) >> 6)] &= ~(1ULL << ((h2 - 1) & 63));
+}
+
+// LDS state of one block
+struct BlockLds {
+  uint32_t *ban1, *ban2;  // M*M bits each, cell (i,j) -> bit (i-1)+M*(j-1)
+  uint32_t *borb;         // 2 * 8 words: bannedOrb per spin (1-based orbitals)
+  double *fock;           // 2*(M+1): fock_diag_tmp(s, i) at [(s-1) + 2*(i-1)]
+  double *mat;            // NS*M*M
+  int *flag;
+};
+
+__device__ inline bool ban_get(const uint32_t *b, int M, int i, int j) {
+  int x = (i - 1) + M * (j - 1);
+  return (b[x >> 5] >> (x & 31)) & 1;
+}
+__device__ inline void ban_set(uint32_t *b, int M, int i, int j) {
+  int x = (i - 1) + M * (j - 1);
+  atomicOr(&b[x >> 5], 1u << (x & 31));
+}
+__device__ inline bool borb_get(const uint32_t *b, int o, int s) {
+  return (b[(s - 1) * 8 + ((o - 1) >> 5)] >> ((o - 1) & 31)) & 1;
+}
+__device__ inline void borb_set(uint32_t *b, int o, int s) {
+  atomicOr(&b[(s - 1) * 8 + ((o - 1) >> 5)], 1u << ((o - 1) & 31));
+}
+
+// owner of mat cell (i,j) (1-based): thread (i+j-2) mod SB
+__device__ inline bool owns(int i, int j) { return ((i + j - 2) & (SB - 1)) == (int)threadIdx.x; }
+
+template <int NI>
+struct Ctx {
+  const Tabs &T;
+  const SelArgs &A;
+  BlockLds &L;
+  const uint64_t *mask;
+  int M, NS;
+  int64_t gath;
+  __device__ Ctx(const Tabs &t, const SelArgs &a, BlockLds &l, const uint64_t *m)
+      : T(t), A(a), L(l), mask(m), M(t.M), NS(a.NS), gath(0) {}
+  __device__ inline void add(int i, int j, const double *coefs, double hij) {
+    double *c = L.mat + (int64_t)NS * ((i - 1) + M * (j - 1));
+    for (int st = 0; st < NS; ++st) c[st] += coefs[st] * hij;
+  }
+  __device__ inline double I(int i, int j, int k, int l) {
+    gath++;
+    return I4(T, i, j, k, l);
+  }
+};
+
+template <int NI>
+__device__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const int (*h)[2], const int (*p)[4], const int *nh,
+                       const int *np, int sp, const double *coefs) {
+  // p[s][k] = k+1-th particle of spin s+1; h similarly
+#define P(n, s) p[(s) - 1][(n) - 1]
+#define H(n, s) h[(s) - 1][(n) - 1]
+  int bant = 1;
+  int tip = np[0] * np[1];
+  int ma = sp;
+  if (np[0] > np[1]) ma = 1;
+  if (np[0] < np[1]) ma = 2;
+  int mi = (ma % 2) + 1;
+  const uint32_t *B1 = C.L.ban1;
+  const int M = C.M;
+  if (sp == 3) {
+    if (ma == 2) bant = 2;
+    const uint32_t *BB = bant == 1 ? C.L.ban1 : C.L.ban2;
+    if (tip == 3) {
+      int puti = P(1, mi);
+      for (int i = 1; i <= 3; ++i) {
+        int putj = P(i, ma);
+        int ci = ma == 1 ? putj : puti, cj = ma == 1 ? puti : putj;
+        if (!owns(ci, cj)) continue;
+        if (ban_get(BB, M, putj, puti)) continue;
+        int i1 = (i == 1) ? 2 : 1, i2 = (i == 3) ? 2 : 3;
+        int p1 = P(i1, ma), p2 = P(i2, ma), h1 = H(1, ma), h2 = H(2, ma);
+        double hij = (C.I(p1, p2, h1, h2) - C.I(p2, p1, h1, h2)) * phase_bi<NI>(sel, ma, ma, h1, p1, h2, p2);
+        C.add(ci, cj, coefs, hij);
+      }
+    } else {
+      int h1 = H(1, 1), h2 = H(1, 2);
+      for (int j = 1; j <= 2; ++j) {
+        int putj = P(j, 2), p2 = P(3 - j, 2);
+        for (int i = 1; i <= 2; ++i) {
+          int puti = P(i, 1);
+          if (!owns(puti, putj)) continue;
+          if (ban_get(BB, M, puti, putj)) continue;
+          int p1 = P(3 - i, 1);
+          double hij = C.I(p1, p2, h1, h2) * phase_bi<NI>(sel, 1, 2, h1, p1, h2, p2);
+          C.add(puti, putj, coefs, hij);
+        }
+      }
+    }
+  } else {
+    if (tip == 0) {
+      int h1 = H(1, ma), h2 = H(2, ma);
+      for (int i = 1; i <= 3; ++i) {
+        int puti = P(i, ma);
+        for (int j = i + 1; j <= 4; ++j) {
+          int putj = P(j, ma);
+          if (!owns(puti, putj)) continue;
+          if (ban_get(B1, M, puti, putj)) continue;
+          int c[2], n = 0;
+          for (int k = 1; k <= 4; ++k)
+            if (k != i && k != j) c[n++] = k;
+          int p1 = P(c[0], ma), p2 = P(c[1], ma);
+          double hij = (C.I(p1, p2, h1, h2) - C.I(p2, p1, h1, h2)) * phase_bi<NI>(sel, ma, ma, h1, p1, h2, p2);
+          C.add(puti, putj, coefs, hij);
+        }
+      }
+    } else if (tip == 3) {
+      int h1 = H(1, mi), h2 = H(1, ma), p1 = P(1, mi);
+      for (int i = 1; i <= 3; ++i) {
+        int i1 = (i == 1) ? 2 : 1, i2 = (i == 3) ? 2 : 3;
+        int puti = P(i1, ma), putj = P(i2, ma);
+        int ci = min(puti, putj), cj = max(puti, putj);
+        if (!owns(ci, cj)) continue;
+        if (ban_get(B1, M, puti, putj)) continue;
+        int p2 = P(i, ma);
+        double hij = C.I(p1, p2, h1, h2) * phase_bi<NI>(sel, mi, ma, h1, p1, h2, p2);
+        C.add(ci, cj, coefs, hij);
+      }
+    } else {
+      int puti = P(1, sp), putj = P(2, sp);
+      if (owns(puti, putj) && !ban_get(B1, M, puti, putj)) {
+        int p1 = P(1, mi), p2 = P(2, mi), h1 = H(1, mi), h2 = H(2, mi);
+        double hij = (C.I(p1, p2, h1, h2) - C.I(p2, p1, h1, h2)) * phase_bi<NI>(sel, mi, mi, h1, p1, h2, p2);
+        C.add(puti, putj, coefs, hij);
+      }
+    }
+  }
+}
+
+// the unique j (1..M) such that thread owns (i, j); 0 if none
+__device__ inline int owned_j(int i, int M) {
+  int j = ((int)threadIdx.x - (i - 1)) & (SB - 1);
+  return (j < M) ? j + 1 : 0;
+}
+
+template <int NI>
+__device__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const int (*h)[2], const int (*p)[4], const int *nh,
+                       const int *np, int sp, const double *coefs, int na_e, int nb_e) {
+  const int M = C.M;
+  const uint32_t *BO = C.L.borb;
+  const uint32_t *B1 = C.L.ban1;
+  // lbanned = bannedOrb + the selector's particles
+  auto lbanned = [&](int o, int s) -> bool {
+    if (borb_get(BO, o, s)) return true;
+    for (int k = 1; k <= np[s - 1]; ++k)
+      if (P(k, s) == o) return true;
+    return false;
+  };
+  int ma = 1;
+  if (np[1] >= 2) ma = 2;
+  int mi = (ma == 1) ? 2 : 1;
+  int bant = 1;
+  if (sp == 3) {
+    if (ma == 2) bant = 2;
+    const uint32_t *BB = bant == 1 ? C.L.ban1 : C.L.ban2;
+    int puti = P(1, mi), hfix = H(1, ma), p1 = P(1, ma), p2 = P(2, ma);
+    if (!borb_get(BO, puti, mi)) {
+      // cells (putj, puti) if ma == 1 else (puti, putj): owned putj is unique
+      int putj = (ma == 1) ? 0 : owned_j(puti, M);
+      if (ma == 1) {
+        int j = ((int)threadIdx.x - (puti - 1)) & (SB - 1);  // i-1 for cell (i, puti)
+        putj = (j < M) ? j + 1 : 0;
+      }
+      if (putj && putj != hfix && !lbanned(putj, ma) && !ban_get(BB, M, putj, puti)) {
+        double hij;
+        if (putj < hfix)
+          hij = (C.I(p1, p2, putj, hfix) - C.I(p2, p1, putj, hfix)) * phase_bi<NI>(sel, ma, ma, putj, p1, hfix, p2);
+        else
+          hij = (C.I(p1, p2, hfix, putj) - C.I(p2, p1, hfix, putj)) * phase_bi<NI>(sel, ma, ma, hfix, p1, putj, p2);
+        if (ma == 1) C.add(putj, puti, coefs, hij);
+        else C.add(puti, putj, coefs, hij);
+      }
+    }
+    int pfix = P(1, mi);
+    // mi part: lines with fixed ma-orbital p1 and p2, varying mi-orbital
+    for (int which = 0; which < 2; ++which) {
+      int pf = which == 0 ? p1 : p2;        // fixed (ma) orbital of the cell
+      int other = which == 0 ? p2 : p1;     // the ma particle in the integral
+      int pu;
+      if (mi == 1) {  // cells (pu, pf)
+        int j = ((int)threadIdx.x - (pf - 1)) & (SB - 1);
+        pu = (j < M) ? j + 1 : 0;
+      } else {        // cells (pf, pu)
+        pu = owned_j(pf, M);
+      }
+      if (!pu || lbanned(pu, mi)) continue;
+      if (ban_get(BB, M, pf, pu)) continue;
+      double hij = C.I(other, pfix, hfix, pu) * phase_bi<NI>(sel, ma, mi, hfix, other, pu, pfix);
+      if (mi == 1) C.add(pu, pf, coefs, hij);
+      else C.add(pf, pu, coefs, hij);
+    }
+  } else {
+    if (np[ma - 1] == 3) {
+      int hfix = H(1, ma);
+      for (int i = 1; i <= 3; ++i) {
+        int puti = P(i, ma);
+        int i1 = (i == 1) ? 2 : 1, i2 = (i == 3) ? 2 : 3;
+        int p1 = P(i1, ma), p2 = P(i2, ma);
+        // cells {x, puti}: (x, puti) for x < puti, (puti, x) for x >= puti
+        for (int half = 0; half < 2; ++half) {
+          int x;
+          if (half == 0) {
+            int j = ((int)threadIdx.x - (puti - 1)) & (SB - 1);
+            x = (j < M) ? j + 1 : 0;
+            if (!x || x >= puti) continue;
+          } else {
+            x = owned_j(puti, M);
+            if (!x || x < puti) continue;
+          }
+          int putj = x;
+          if (putj == hfix) continue;
+          if (lbanned(putj, ma) || ban_get(B1, M, puti, putj)) continue;
+          double hij;
+          if (putj < hfix)
+            hij = (C.I(p1, p2, putj, hfix) - C.I(p2, p1, putj, hfix)) * phase_bi<NI>(sel, ma, ma, putj, p1, hfix, p2);
+          else
+            hij = (C.I(p1, p2, hfix, putj) - C.I(p2, p1, hfix, putj)) * phase_bi<NI>(sel, ma, ma, hfix, p1, putj, p2);
+          if (half == 0) C.add(x, puti, coefs, hij);
+          else C.add(puti, x, coefs, hij);
+        }
+      }
+    } else {
+      int hfix = H(1, mi), pfix = P(1, mi), p1 = P(1, ma), p2 = P(2, ma);
+      for (int which = 0; which < 2; ++which) {
+        int pc = which == 0 ? p2 : p1;     // line {x, pc}
+        int other = which == 0 ? p1 : p2;
+        for (int half = 0; half < 2; ++half) {
+          int x;
+          if (half == 0) {
+            int j = ((int)threadIdx.x - (pc - 1)) & (SB - 1);
+            x = (j < M) ? j + 1 : 0;
+            if (!x || x >= pc) continue;
+          } else {
+            x = owned_j(pc, M);
+            if (!x || x < pc) continue;
+          }
+          int puti = x;
+          if (lbanned(puti, ma)) continue;
+          if (ban_get(B1, M, puti, pc)) continue;
+          double hij = C.I(pfix, other, hfix, puti) * phase_bi<NI>(sel, mi, ma, hfix, pfix, puti, other);
+          if (half == 0) C.add(x, pc, coefs, hij);
+          else C.add(pc, x, coefs, hij);
+        }
+      }
+    }
+  }
+  // singles folded into this block
+  int s1, s2;
+  if (sp == 3) { s1 = 1; s2 = 2; } else { s1 = sp; s2 = sp; }
+  for (int i1 = 1; i1 <= np[s1 - 1]; ++i1) {
+    int ib = (s1 == s2) ? i1 + 1 : 1;
+    for (int i2 = ib; i2 <= np[s2 - 1]; ++i2) {
+      int p1 = P(i1, s1), p2 = P(i2, s2);
+      if (!owns(p1, p2)) continue;
+      if (borb_get(BO, p1, s1) || borb_get(BO, p2, s2) || ban_get(B1, M, p1, p2)) continue;
+      uint64_t det[2 * NI];
+#pragma unroll
+      for (int w = 0; w < 2 * NI; ++w) det[w] = C.mask[w];
+      det[(s1 - 1) * NI + ((p1 - 1) >> 6)] |= 1ULL << ((p1 - 1) & 63);
+      det[(s2 - 1) * NI + ((p2 - 1) >> 6)] |= 1ULL << ((p2 - 1) & 63);
+      double hij = i_H_j<NI>(C.T, sel, det, na_e, nb_e);
+      C.gath += 4;
+      C.add(p1, p2, coefs, hij);
+    }
+  }
+}
+
+template <int NI>
+__device__ void get_d0(Ctx<NI> &C, const uint64_t *sel, const int (*p)[4], int sp, const double *coefs, int na_e,
+                       int nb_e) {
+  const int M = C.M;
+  const uint32_t *BO = C.L.borb;
+  const uint32_t *B1 = C.L.ban1;
+  uint64_t det[2 * NI];
+  if (sp == 3) {
+    int h1 = P(1, 1), h2 = P(1, 2);
+    for (int p1 = 1; p1 <= M; ++p1) {
+      int p2 = owned_j(p1, M);
+      if (!p2) continue;
+      if (borb_get(BO, p1, 1) || borb_get(BO, p2, 2) || ban_get(B1, M, p1, p2)) continue;
+      double hij;
+      if (p1 == h1 || p2 == h2) {
+#pragma unroll
+        for (int w = 0; w < 2 * NI; ++w) det[w] = C.mask[w];
+        det[(p1 - 1) >> 6] |= 1ULL << ((p1 - 1) & 63);
+        det[NI + ((p2 - 1) >> 6)] |= 1ULL << ((p2 - 1) & 63);
+        hij = i_H_j<NI>(C.T, sel, det, na_e, nb_e);
+        C.gath += 4;
+      } else {
+        hij = C.I(p1, p2, h1, h2) * phase_bi<NI>(sel, 1, 2, h1, p1, h2, p2);
+      }
+      C.add(p1, p2, coefs, hij);
+    }
+  } else {
+    int p1 = P(1, sp), p2 = P(2, sp);
+    for (int puti = 1; puti <= M; ++puti) {
+      int putj = owned_j(puti, M);
+      if (!putj || putj <= puti) continue;
+      if (borb_get(BO, puti, sp) || borb_get(BO, putj, sp) || ban_get(B1, M, puti, putj)) continue;
+      double hij;
+      if (puti == p1 || putj == p2 || puti == p2 || putj == p1) {
+#pragma unroll
+        for (int w = 0; w < 2 * NI; ++w) det[w] = C.mask[w];
+        det[(sp - 1) * NI + ((puti - 1) >> 6)] |= 1ULL << ((puti - 1) & 63);
+        det[(sp - 1) * NI + ((putj - 1) >> 6)] |= 1ULL << ((putj - 1) & 63);
+        hij = i_H_j<NI>(C.T, sel, det, na_e, nb_e);
+        C.gath += 4;
+      } else {
+        hij = (C.I(p1, p2, puti, putj) - C.I(p2, p1, puti, putj)) * phase_bi<NI>(sel, sp, sp, puti, p1, putj, p2);
+      }
+      C.add(puti, putj, coefs, hij);
+    }
+  }
+#undef P
+#undef H
+}
+
+template <int NI>
+__device__ inline void sel_lists(const uint64_t *mask, const uint64_t *d, int (*p)[4], int *np, int (*h)[2], int *nh) {
+  for (int s = 0; s < 2; ++s) {
+    np[s] = 0;
+    nh[s] = 0;
+#pragma unroll
+    for (int w = 0; w < NI; ++w) {
+      uint64_t mob = ~mask[s * NI + w] & d[s * NI + w];
+      uint64_t per = mask[s * NI + w] & ~d[s * NI + w];
+      while (mob) {
+        int o = 64 * w + __ffsll((unsigned long long)mob);
+        mob &= mob - 1;
+        if (np[s] < 4) p[s][np[s]] = o;
+        np[s]++;
+      }
+      while (per) {
+        int o = 64 * w + __ffsll((unsigned long long)per);
+        per &= per - 1;
+        if (nh[s] < 2) h[s][nh[s]] = o;
+        nh[s]++;
+      }
+    }
+  }
+}
+
+// diag_H_mat_elem_fock (slater_rules.irp.f:1800-1867)
+__device__ inline double diag_fock(const Tabs &T, const double *f, int sp, int h1, int p1, int h2, int p2) {
+  const int M = T.M;
+#define F(s, i) f[((s) - 1) + 2 * ((i) - 1)]
+#define JA(a, b) T.JA[((a) - 1) + M * ((b) - 1)]
+#define JJ(a, b) T.J[((a) - 1) + M * ((b) - 1)]
+  double E0 = F(1, M + 1);
+  // degree and ordering as get_double_excitation(gen, det) would return
+  if (sp == 3) {
+    bool da = (p1 != h1), db = (p2 != h2);
+    if (da && db) {
+      return E0 - F(1, h1) + (F(1, p1) - JA(h1, p1)) - (F(2, h2) - JJ(h1, h2) + JJ(p1, h2)) +
+             (F(2, p2) - JJ(h1, p2) + JJ(p1, p2) - JA(h2, p2));
+    }
+    if (da) return E0 - F(1, h1) + (F(1, p1) - JA(h1, p1));
+    if (db) return E0 - F(2, h2) + (F(2, p2) - JA(h2, p2));
+    return E0;
+  }
+  // same spin: holes {h1 < h2}, particles {p1 < p2}
+  int hs[2] = {h1, h2}, ps[2] = {p1, p2};
+  int hr[2], pr[2], n = 0;
+  bool usedp[2] = {false, false}, usedh[2] = {false, false};
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      if (!usedh[a] && !usedp[b] && hs[a] == ps[b]) { usedh[a] = true; usedp[b] = true; }
+  int k = 0;
+  for (int a = 0; a < 2; ++a) if (!usedh[a]) hr[k++] = hs[a];
+  n = k;
+  k = 0;
+  for (int b = 0; b < 2; ++b) if (!usedp[b]) pr[k++] = ps[b];
+  const int s = sp;
+  if (n == 2) {
+    return E0 - F(s, hr[0]) + (F(s, pr[0]) - JA(hr[0], pr[0])) - (F(s, hr[1]) - JA(hr[0], hr[1]) + JA(pr[0], hr[1])) +
+           (F(s, pr[1]) - JA(hr[0], pr[1]) + JA(pr[0], pr[1]) - JA(hr[1], pr[1]));
+  }
+  if (n == 1) return E0 - F(s, hr[0]) + (F(s, pr[0]) - JA(hr[0], pr[0]));
+  return E0;
+#undef F
+#undef JA
+#undef JJ
+}
+
+template <int NI>
+__global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int M = T.M, NS = A.NS;
+  const int nbits = (M * M + 31) / 32;
+  BlockLds L;
+  L.mat = reinterpret_cast<double *>(lds_raw);
+  L.fock = L.mat + (size_t)NS * M * M;
+  L.ban1 = reinterpret_cast<uint32_t *>(L.fock + 2 * (M + 1));
+  L.ban2 = L.ban1 + nbits;
+  L.borb = L.ban2 + nbits;
+  L.flag = reinterpret_cast<int *>(L.borb + 16);
+  double *red = reinterpret_cast<double *>(L.flag + 4);  // SB * MAXNS doubles (8-aligned below)
+  red = reinterpret_cast<double *>((reinterpret_cast<uintptr_t>(red) + 7) & ~uintptr_t(7));
+
+  const BlockDesc B = A.blocks[blockIdx.x];
+  const int tid = threadIdx.x;
+  const uint64_t *gen = A.psi + (int64_t)(B.gen - 1) * 2 * NI;
+  const int sp = (B.s1 != B.s2) ? 3 : B.s1;
+  uint64_t mask[2 * NI];
+  build_mask<NI>(gen, B.s1, B.h1, B.s2, B.h2, mask);
+
+  // ---- init ----
+  for (int x = tid; x < NS * M * M; x += SB) L.mat[x] = 0.0;
+  for (int x = tid; x < 2 * nbits; x += SB) L.ban1[x] = 0u;  // ban1 and ban2 are contiguous
+  if (tid < 16) {
+    // bannedOrb: every orbital occupied in the generator (holes of the
+    // particle mask), then the mono unbans
+    int s = tid / 8 + 1, w = tid & 7;
+    uint32_t v = 0;
+    for (int bb = 0; bb < 32; ++bb) {
+      int o = w * 32 + bb + 1;
+      bool banned = (o > M) || bit(gen + (s - 1) * NI, o);
+      if (banned) v |= 1u << bb;
+    }
+    L.borb[tid] = v;
+  }
+  if (tid == 0) L.flag[0] = 0;
+  // fock_diag_tmp (Fock_diag.irp.f:1-93)
+  {
+    const uint64_t *ga = gen, *gb = gen + NI;
+#define F(s, i) L.fock[((s) - 1) + 2 * ((i) - 1)]
+#define JA(a, b) T.JA[((a) - 1) + M * ((b) - 1)]
+#define JJ(a, b) T.J[((a) - 1) + M * ((b) - 1)]
+    for (int x = tid; x < 2 * M; x += SB) {
+      int s = x / M + 1, i = x % M + 1;
+      const uint64_t *same = s == 1 ? ga : gb, *oth = s == 1 ? gb : ga;
+      double f = 0.0;
+      if (bit(same, i)) {
+        f += T.hcore[(i - 1) * (M + 1)];
+        for (int w = 0; w < NI; ++w) {
+          uint64_t y = same[w];
+          while (y) { int j = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; if (j != i) f += JA(i, j); }
+        }
+        for (int w = 0; w < NI; ++w) {
+          uint64_t y = oth[w];
+          while (y) { int j = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; f += JJ(i, j); }
+        }
+      }
+      if (f == 0.0) {
+        f += T.hcore[(i - 1) * (M + 1)];
+        for (int w = 0; w < NI; ++w) {
+          uint64_t y = same[w];
+          while (y) { int j = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; f += JA(i, j); }
+        }
+        for (int w = 0; w < NI; ++w) {
+          uint64_t y = oth[w];
+          while (y) { int j = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; f += JJ(i, j); }
+        }
+      }
+      F(s, i) = f;
+    }
+    if (tid == 0) {
+      double E0 = 0.0;
+      for (int w = 0; w < NI; ++w) {
+        uint64_t y = ga[w];
+        while (y) {
+          int i = 64 * w + __ffsll((unsigned long long)y);
+          y &= y - 1;
+          E0 += T.hcore[(i - 1) * (M + 1)];
+          for (int w2 = 0; w2 < NI; ++w2) {
+            uint64_t z = ga[w2];
+            while (z) { int j = 64 * w2 + __ffsll((unsigned long long)z); z &= z - 1; if (j != i) E0 += 0.5 * JA(i, j); }
+          }
+          for (int w2 = 0; w2 < NI; ++w2) {
+            uint64_t z = gb[w2];
+            while (z) { int j = 64 * w2 + __ffsll((unsigned long long)z); z &= z - 1; E0 += JJ(i, j); }
+          }
+        }
+      }
+      for (int w = 0; w < NI; ++w) {
+        uint64_t y = gb[w];
+        while (y) {
+          int i = 64 * w + __ffsll((unsigned long long)y);
+          y &= y - 1;
+          E0 += T.hcore[(i - 1) * (M + 1)];
+          for (int w2 = 0; w2 < NI; ++w2) {
+            uint64_t z = gb[w2];
+            while (z) { int j = 64 * w2 + __ffsll((unsigned long long)z); z &= z - 1; if (j != i) E0 += 0.5 * JA(i, j); }
+          }
+        }
+      }
+      F(1, M + 1) = E0;
+      F(2, M + 1) = E0;
+    }
+#undef F
+#undef JA
+#undef JJ
+  }
+  __syncthreads();
+  if (tid == 0) {
+    if (B.unbanA) L.borb[(B.s1 - 1) * 8 + ((B.h1 - 1) >> 5)] &= ~(1u << ((B.h1 - 1) & 31));
+    if (B.unbanB) L.borb[(B.s2 - 1) * 8 + ((B.h2 - 1) >> 5)] &= ~(1u << ((B.h2 - 1) & 31));
+  }
+  __syncthreads();
+
+  const int *near = A.near_list + A.near_off[B.gslot];
+  const int n_near = A.near_cnt[B.gslot];
+
+  // ---- spot_isinwf ----
+  for (int x = tid; x < n_near; x += SB) {
+    const int idx = near[x];
+    const uint64_t *d = A.psi + (int64_t)(idx - 1) * 2 * NI;
+    bool cont = true;
+#pragma unroll
+    for (int w = 0; w < 2 * NI; ++w) cont = cont && ((d[w] & mask[w]) == mask[w]);
+    if (!cont) continue;
+    if (idx < B.gen) {
+      atomicOr(&L.flag[0], 1);
+      continue;
+    }
+    int lst[2], n = 0;
+#pragma unroll
+    for (int w = 0; w < 2 * NI; ++w) {
+      uint64_t y = d[w] & ~mask[w];
+      while (y && n < 2) {
+        lst[n++] = 64 * (w % NI) + __ffsll((unsigned long long)y);
+        y &= y - 1;
+      }
+    }
+    if (n == 2) ban_set(L.ban1, M, lst[0], lst[1]);
+  }
+  __syncthreads();
+  if (L.flag[0]) {
+    if (tid == 0) {
+      A.blk_status[blockIdx.x] = 1;
+      for (int st = 0; st < NS; ++st) A.blk_pt2[(int64_t)blockIdx.x * NS + st] = 0.0;
+      A.blk_ncand[blockIdx.x] = 0;
+      A.blk_ngath[blockIdx.x] = 0;
+    }
+    return;
+  }
+  // ---- past_d1 / past_d2 (selectors before the generator) ----
+  for (int x = tid; x < n_near; x += SB) {
+    const int idx = near[x];
+    if (idx >= B.gen || idx > A.n_sel) continue;
+    const uint64_t *d = A.psi + (int64_t)(idx - 1) * 2 * NI;
+    int nt = 0;
+#pragma unroll
+    for (int w = 0; w < 2 * NI; ++w) nt += __popcll(~mask[w] & d[w]);
+    if (nt < 3 || nt > 4) continue;
+    int p[2][4], np[2], h[2][2], nh[2];
+    sel_lists<NI>(mask, d, p, np, h, nh);
+    if (nt == 4) {
+      if (sp == 3) {
+        for (int i = 0; i < np[0]; ++i)
+          for (int j = 0; j < np[1]; ++j) ban_set(L.ban1, M, p[0][i], p[1][j]);
+      } else {
+        for (int i = 0; i < np[sp - 1]; ++i)
+          for (int j = 0; j < i; ++j) {
+            ban_set(L.ban1, M, p[sp - 1][j], p[sp - 1][i]);
+            ban_set(L.ban1, M, p[sp - 1][i], p[sp - 1][j]);
+          }
+      }
+    } else {
+      for (int s = 0; s < 2; ++s)
+        for (int i = 0; i < np[s]; ++i) borb_set(L.borb, p[s][i], s + 1);
+    }
+  }
+  __syncthreads();
+  // ---- symmetrisation when the generator itself is met (splash_pq:692-706) ----
+  if (sp == 3) {
+    for (int x = tid; x < M * M; x += SB) {
+      int j = x % M + 1, k = x / M + 1;
+      if (ban_get(L.ban1, M, k, j)) ban_set(L.ban2, M, j, k);
+    }
+  } else {
+    for (int x = tid; x < M * M; x += SB) {
+      int k = x % M + 1, l = x / M + 1;
+      if (l > k && ban_get(L.ban1, M, k, l)) ban_set(L.ban1, M, l, k);
+    }
+  }
+  __syncthreads();
+  // lower-triangle entries that the reference overwrites with a false upper
+  // entry: banned(l,k,1) = banned(k,l,1) also clears bits set only below the
+  // diagonal by past_d2 (never happens: past_d2 sets both orientations)
+
+  // ---- splash_pq: couplings in selector order ----
+  Ctx<NI> C(T, A, L, mask);
+  for (int x = 0; x < n_near; ++x) {
+    const int idx = near[x];
+    if (idx < B.gen || idx > A.n_sel) continue;
+    const uint64_t *d = A.psi + (int64_t)(idx - 1) * 2 * NI;
+    int nt = 0;
+#pragma unroll
+    for (int w = 0; w < 2 * NI; ++w) nt += __popcll(~mask[w] & d[w]);
+    if (nt > 4) continue;
+    int p[2][4], np[2], h[2][2], nh[2];
+    sel_lists<NI>(mask, d, p, np, h, nh);
+    double coefs[MAXNS];
+    for (int st = 0; st < NS; ++st) coefs[st] = A.coef[(idx - 1) + A.n_det * st];
+    if (tid == 0) C.gath += NS;
+    if (nt == 4) get_d2<NI>(C, d, h, p, nh, np, sp, coefs);
+    else if (nt == 3) get_d1<NI>(C, d, h, p, nh, np, sp, coefs, A.na, A.nb);
+    else get_d0<NI>(C, d, p, sp, coefs, A.na, A.nb);
+  }
+  __syncthreads();
+
+  // ---- fill_buffer_double ----
+  const int s1 = (sp == 3) ? 1 : sp, s2 = (sp == 3) ? 2 : sp;
+  double pt2[MAXNS];
+  for (int st = 0; st < NS; ++st) pt2[st] = 0.0;
+  int64_t ncand = 0;
+  bool overflow = false;
+  for (int p1 = 1; p1 <= M; ++p1) {
+    int p2 = owned_j(p1, M);
+    if (!p2) continue;
+    if (sp != 3 && p2 <= p1) continue;
+    if (borb_get(L.borb, p1, s1) || borb_get(L.borb, p2, s2)) continue;
+    if (ban_get(L.ban1, M, p1, p2)) continue;
+    const double *m = L.mat + (int64_t)NS * ((p1 - 1) + M * (p2 - 1));
+    double sa = 0.0;
+    for (int st = 0; st < NS; ++st) sa += fabs(m[st]);
+    if (sa == 0.0) continue;
+    double Hii = diag_fock(T, L.fock, sp, B.h1, p1, B.h2, p2);
+    double min_e = 0.0;
+    for (int st = 0; st < NS; ++st) {
+      double dE = A.E0[st] - Hii;
+      double val = m[st] + m[st];
+      double tmp = sqrt(dE * dE + val * val);
+      if (dE < 0.0) tmp = -tmp;
+      double e = 0.5 * (tmp - dE);
+      pt2[st] += e;
+      min_e = min(e, min_e);
+    }
+    ++ncand;
+    if (min_e <= A.tau) {
+      unsigned long long slot = atomicAdd(A.pool_count, 1ULL);
+      if ((int64_t)slot < A.pool_cap) {
+        uint64_t det[2 * NI];
+#pragma unroll
+        for (int w = 0; w < 2 * NI; ++w) det[w] = mask[w];
+        det[(s1 - 1) * NI + ((p1 - 1) >> 6)] |= 1ULL << ((p1 - 1) & 63);
+        det[(s2 - 1) * NI + ((p2 - 1) >> 6)] |= 1ULL << ((p2 - 1) & 63);
+        A.pool_key[slot] = min_e;
+#pragma unroll
+        for (int w = 0; w < 2 * NI; ++w) A.pool_det[slot * 2 * NI + w] = det[w];
+        A.pool_blk[slot] = A.blk_base + blockIdx.x;
+      } else {
+        overflow = true;
+      }
+    }
+  }
+  // ---- block reductions (fixed tree order) ----
+  __syncthreads();
+  if (tid == 0) L.flag[1] = 0;
+  __syncthreads();
+  if (overflow) atomicOr(&L.flag[1], 1);
+  for (int st = 0; st < NS; ++st) red[st * SB + tid] = pt2[st];
+  __syncthreads();
+  for (int off = SB / 2; off > 0; off >>= 1) {
+    if (tid < off)
+      for (int st = 0; st < NS; ++st) red[st * SB + tid] += red[st * SB + tid + off];
+    __syncthreads();
+  }
+  if (tid == 0)
+    for (int st = 0; st < NS; ++st) A.blk_pt2[(int64_t)blockIdx.x * NS + st] = red[st * SB];
+  __syncthreads();
+  // counters: candidates and gathers
+  int64_t *ired = reinterpret_cast<int64_t *>(red);
+  ired[tid] = ncand;
+  __syncthreads();
+  for (int off = SB / 2; off > 0; off >>= 1) {
+    if (tid < off) ired[tid] += ired[tid + off];
+    __syncthreads();
+  }
+  if (tid == 0) A.blk_ncand[blockIdx.x] = ired[0];
+  __syncthreads();
+  ired[tid] = C.gath;
+  __syncthreads();
+  for (int off = SB / 2; off > 0; off >>= 1) {
+    if (tid < off) ired[tid] += ired[tid + off];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    A.blk_ngath[blockIdx.x] = ired[0];
+    A.blk_status[blockIdx.x] = L.flag[1] ? 2 : 0;
+  }
+}
+
+// drop entries from given blocks and entries above tau; returns via count
+template <int NI>
+__global__ void pool_compact_kernel(const double *__restrict__ key, const uint64_t *__restrict__ det,
+                                    const int *__restrict__ blk, int64_t n, double tau,
+                                    const uint8_t *__restrict__ drop_blk, double *__restrict__ key2,
+                                    uint64_t *__restrict__ det2, int *__restrict__ blk2,
+                                    unsigned long long *__restrict__ count2) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (key[i] > tau) continue;
+    if (drop_blk && drop_blk[blk[i]]) continue;
+    unsigned long long s = atomicAdd(count2, 1ULL);
+    key2[s] = key[i];
+    for (int w = 0; w < 2 * NI; ++w) det2[s * 2 * NI + w] = det[i * 2 * NI + w];
+    blk2[s] = blk[i];
+  }
+}
+
+}  // namespace
+
+// --------------------------------------------------------------------------
+// host driver
+// --------------------------------------------------------------------------
+namespace {
+
+struct Pool {
+  DBuf<double> key, key2;
+  DBuf<uint64_t> det, det2;
+  DBuf<int> blk, blk2;
+  DBuf<unsigned long long> count;
+  int64_t cap = 0;
+};
+
+template <int NI>
+void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel, const double *E0, int64_t n_keep,
+                   int subset, double *pt2_out, uint64_t *out_dets, double *out_vals, int64_t *n_out,
+                   const uint8_t *keep_mask, double *block_pt2_out, int64_t *block_info_out) {
+  Wfn &W = c->W;
+  Ham &H = c->H;
+  hipStream_t st = c->stream;
+  const int64_t n_det = W.n_det;
+  const int M = H.M;
+  // LDS budget
+  const int nbits = (M * M + 31) / 32;
+  size_t lds = sizeof(double) * ((size_t)NS * M * M + 2 * (M + 1)) + sizeof(uint32_t) * (2 * nbits + 16) +
+               sizeof(int) * 4 + 8 + sizeof(double) * SB * std::max(NS, 1);
+  QPX_REQUIRE(lds <= (size_t)c->lds_per_block,
+              "selection: N_states * mo_num^2 coupling matrix does not fit in LDS of one workgroup");
+  auto kfun = select_block_kernel<NI>;
+  QPX_HIP(hipFuncSetAttribute((const void *)kfun, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+
+  DBuf<double> coef_d;
+  coef_d.upload(coef, (size_t)n_det * NS, st);
+  Tabs T{H.V.p, H.hcore.p, H.Jd.p, H.JAd.p, H.fock_cs.p, M, H.npair, H.ref_energy};
+
+  // hole lists = occupied orbitals of each generator (full generator masks)
+  std::vector<uint64_t> psi_h((size_t)n_det * 2 * NI);
+  QPX_HIP(hipMemcpyAsync(psi_h.data(), W.psi.p, sizeof(uint64_t) * psi_h.size(), hipMemcpyDeviceToHost, st));
+  QPX_HIP(hipStreamSynchronize(st));
+
+  const int fragment_count = H.na * H.na;
+  // all blocks in generator order
+  std::vector<BlockDesc> all;
+  std::vector<int> gen_first;  // first block of each generator
+  for (int g = 1; g <= n_gen; ++g) {
+    gen_first.push_back((int)all.size());
+    const uint64_t *gd = &psi_h[(size_t)(g - 1) * 2 * NI];
+    std::vector<int> hl[2];
+    for (int s = 0; s < 2; ++s)
+      for (int w = 0; w < NI; ++w) {
+        uint64_t y = gd[s * NI + w];
+        while (y) { hl[s].push_back(64 * w + __builtin_ctzll(y) + 1); y &= y - 1; }
+      }
+    int maskInd = -1;
+    bool monoB = true;
+    for (int s1 = 1; s1 <= 2; ++s1)
+      for (int i1 = (int)hl[s1 - 1].size(); i1 >= 1; --i1) {
+        int h1 = hl[s1 - 1][i1 - 1];
+        for (int s2 = s1; s2 <= 2; ++s2) {
+          int ib = (s1 == s2) ? i1 + 1 : 1;
+          bool monoA = true;
+          for (int i2 = (int)hl[s2 - 1].size(); i2 >= ib; --i2) {
+            int h2 = hl[s2 - 1][i2 - 1];
+            BlockDesc b{};
+            b.gen = g;
+            b.s1 = (int8_t)s1; b.s2 = (int8_t)s2;
+            b.h1 = (int16_t)h1; b.h2 = (int16_t)h2;
+            b.unbanA = (s1 != s2 && monoB) ? 1 : 0;
+            b.unbanB = 0;
+            if (s1 != s2 && monoA) { b.unbanB = 1; monoA = false; }
+            ++maskInd;
+            b.mask_ind = maskInd;
+            bool keep = (subset == 0 || (maskInd % fragment_count) == subset - 1);
+            if (keep_mask) keep = keep && keep_mask[all.size()] != 0;
+            b.gslot = keep ? 1 : 0;  // temporary: keep flag
+            all.push_back(b);
+          }
+          if (s1 != s2) monoB = false;
+        }
+      }
+  }
+  gen_first.push_back((int)all.size());
+  const int64_t n_blocks_total = (int64_t)all.size();
+
+  Pool P;
+  P.cap = std::max<int64_t>(8 * n_keep + (1 << 20), 1 << 22);
+  P.key.alloc(P.cap); P.key2.alloc(P.cap);
+  P.det.alloc((size_t)P.cap * 2 * NI); P.det2.alloc((size_t)P.cap * 2 * NI);
+  P.blk.alloc(P.cap); P.blk2.alloc(P.cap);
+  P.count.alloc(2);
+  QPX_HIP(hipMemsetAsync(P.count.p, 0, sizeof(unsigned long long) * 2, st));
+  int64_t pool_n = 0;
+  double tau = 0.0;
+
+  std::vector<double> blk_pt2_all((size_t)n_blocks_total * NS, 0.0);
+  std::vector<int> blk_done(n_blocks_total, 0);
+  int64_t ncand_total = 0, ngath_total = 0, nproc = 0;
+
+  DBuf<int> gens_d, near_cnt_d, near_list_d;
+  DBuf<int64_t> near_off_d;
+  DBuf<BlockDesc> blocks_d;
+  DBuf<double> bpt2_d;
+  DBuf<int> bstat_d;
+  DBuf<int64_t> bncand_d, bngath_d;
+  DBuf<uint8_t> drop_d;
+  std::vector<uint8_t> drop(n_blocks_total, 0);
+
+  auto compact = [&](bool with_drop) {
+    // tighten tau to the n_keep-th best key (ties kept), drop entries above it
+    // and entries of blocks that will be re-run
+    std::vector<double> keys(pool_n);
+    QPX_HIP(hipMemcpyAsync(keys.data(), P.key.p, sizeof(double) * pool_n, hipMemcpyDeviceToHost, st));
+    QPX_HIP(hipStreamSynchronize(st));
+    std::vector<int> blkh;
+    if (with_drop) {
+      blkh.resize(pool_n);
+      QPX_HIP(hipMemcpyAsync(blkh.data(), P.blk.p, sizeof(int) * pool_n, hipMemcpyDeviceToHost, st));
+      QPX_HIP(hipStreamSynchronize(st));
+      std::vector<double> k2;
+      k2.reserve(pool_n);
+      for (int64_t i = 0; i < pool_n; ++i)
+        if (!drop[blkh[i]]) k2.push_back(keys[i]);
+      keys.swap(k2);
+    }
+    if ((int64_t)keys.size() >= n_keep && n_keep > 0) {
+      std::nth_element(keys.begin(), keys.begin() + (n_keep - 1), keys.end());
+      tau = std::min(tau, keys[n_keep - 1]);
+    }
+    if (with_drop) drop_d.upload(drop, st);
+    QPX_HIP(hipMemsetAsync(P.count.p + 1, 0, sizeof(unsigned long long), st));
+    int grid = (int)std::min<int64_t>((pool_n + 255) / 256, 65536);
+    if (pool_n > 0)
+      hipLaunchKernelGGL(pool_compact_kernel<NI>, dim3(grid), dim3(256), 0, st, P.key.p, P.det.p, P.blk.p, pool_n,
+                         tau, with_drop ? drop_d.p : (const uint8_t *)nullptr, P.key2.p, P.det2.p, P.blk2.p,
+                         P.count.p + 1);
+    QPX_HIP(hipGetLastError());
+    unsigned long long n2 = 0;
+    QPX_HIP(hipMemcpyAsync(&n2, P.count.p + 1, sizeof(n2), hipMemcpyDeviceToHost, st));
+    QPX_HIP(hipStreamSynchronize(st));
+    std::swap(P.key.p, P.key2.p);
+    std::swap(P.det.p, P.det2.p);
+    std::swap(P.blk.p, P.blk2.p);
+    pool_n = (int64_t)n2;
+    unsigned long long pn = (unsigned long long)pool_n;
+    QPX_HIP(hipMemcpyAsync(P.count.p, &pn, sizeof(pn), hipMemcpyHostToDevice, st));
+    QPX_HIP(hipStreamSynchronize(st));
+  };
+
+  // batches of generators: bounded near-list size and block count
+  int g0 = 1;
+  int batch_gens = 1;  // the first generator alone establishes tau
+  while (g0 <= n_gen) {
+    int g1 = std::min(n_gen, g0 + batch_gens - 1);
+    int ng = g1 - g0 + 1;
+    std::vector<int> gens(ng);
+    std::iota(gens.begin(), gens.end(), g0);
+    gens_d.upload(gens, st);
+    near_cnt_d.alloc(ng);
+    near_off_d.alloc(ng);
+    NearArgs NA{W.psi.p, n_det, n_sel, gens_d.p, ng, near_off_d.p, near_cnt_d.p, nullptr};
+    hipLaunchKernelGGL(near_kernel<NI>, dim3(ng), dim3(256), 0, st, NA, 0);
+    QPX_HIP(hipGetLastError());
+    std::vector<int> cnt(ng);
+    QPX_HIP(hipMemcpyAsync(cnt.data(), near_cnt_d.p, sizeof(int) * ng, hipMemcpyDeviceToHost, st));
+    QPX_HIP(hipStreamSynchronize(st));
+    std::vector<int64_t> off(ng);
+    int64_t tot = 0;
+    for (int i = 0; i < ng; ++i) { off[i] = tot; tot += cnt[i]; }
+    near_off_d.upload(off, st);
+    near_list_d.alloc(std::max<int64_t>(tot, 1));
+    NA.list = near_list_d.p;
+    hipLaunchKernelGGL(near_kernel<NI>, dim3(ng), dim3(256), 0, st, NA, 1);
+    QPX_HIP(hipGetLastError());
+    // blocks of this batch
+    std::vector<BlockDesc> bl;
+    std::vector<int64_t> bl_gid;
+    for (int g = g0; g <= g1; ++g)
+      for (int b = gen_first[g - 1]; b < gen_first[g]; ++b) {
+        if (!all[b].gslot) continue;  // filtered out by subset / keep mask
+        BlockDesc d = all[b];
+        d.gslot = g - g0;
+        bl.push_back(d);
+        bl_gid.push_back(b);
+      }
+    std::vector<int64_t> pending(bl.size());
+    std::iota(pending.begin(), pending.end(), 0);
+    while (!pending.empty()) {
+      std::vector<BlockDesc> run(pending.size());
+      for (size_t i = 0; i < pending.size(); ++i) run[i] = bl[pending[i]];
+      const int nb = (int)run.size();
+      blocks_d.upload(run, st);
+      bpt2_d.alloc((size_t)nb * NS);
+      bstat_d.alloc(nb);
+      bncand_d.alloc(nb);
+      bngath_d.alloc(nb);
+      // global ids for the pool: the run index maps back through pending
+      SelArgs SA{};
+      SA.psi = W.psi.p; SA.coef = coef_d.p; SA.n_det = n_det; SA.n_sel = n_sel; SA.NS = NS;
+      SA.na = H.na; SA.nb = H.nb;
+      for (int s = 0; s < NS; ++s) SA.E0[s] = E0[s];
+      SA.blocks = blocks_d.p; SA.n_blocks = nb;
+      SA.near_off = near_off_d.p; SA.near_cnt = near_cnt_d.p; SA.near_list = near_list_d.p;
+      SA.pool_key = P.key.p; SA.pool_det = P.det.p; SA.pool_blk = P.blk.p; SA.pool_count = P.count.p;
+      SA.pool_cap = P.cap; SA.tau = tau;
+      SA.blk_pt2 = bpt2_d.p; SA.blk_status = bstat_d.p; SA.blk_ncand = bncand_d.p; SA.blk_ngath = bngath_d.p;
+      // the pool records run-local block ids; they are translated to global
+      // ids right after the launch
+      SA.blk_base = 0;
+      // remember the mapping run index -> global block id for drops
+      std::vector<int64_t> run_gid(nb);
+      for (int i = 0; i < nb; ++i) run_gid[i] = bl_gid[pending[i]];
+      int64_t pool_before = pool_n;
+      hipLaunchKernelGGL(kfun, dim3(nb), dim3(SB), lds, st, SA, T);
+      QPX_HIP(hipGetLastError());
+      std::vector<double> bp((size_t)nb * NS);
+      std::vector<int> bs(nb);
+      std::vector<int64_t> bc(nb), bg(nb);
+      QPX_HIP(hipMemcpyAsync(bp.data(), bpt2_d.p, sizeof(double) * bp.size(), hipMemcpyDeviceToHost, st));
+      QPX_HIP(hipMemcpyAsync(bs.data(), bstat_d.p, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+      QPX_HIP(hipMemcpyAsync(bc.data(), bncand_d.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, st));
+      QPX_HIP(hipMemcpyAsync(bg.data(), bngath_d.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, st));
+      unsigned long long pc = 0;
+      QPX_HIP(hipMemcpyAsync(&pc, P.count.p, sizeof(pc), hipMemcpyDeviceToHost, st));
+      QPX_HIP(hipStreamSynchronize(st));
+      pool_n = std::min<int64_t>((int64_t)pc, P.cap);
+      // translate run-local block ids stored in the pool to global ids
+      if (pool_n > pool_before) {
+        std::vector<int> pb(pool_n - pool_before);
+        QPX_HIP(hipMemcpyAsync(pb.data(), P.blk.p + pool_before, sizeof(int) * pb.size(), hipMemcpyDeviceToHost, st));
+        QPX_HIP(hipStreamSynchronize(st));
+        for (auto &x : pb) x = (int)run_gid[x];
+        QPX_HIP(hipMemcpyAsync(P.blk.p + pool_before, pb.data(), sizeof(int) * pb.size(), hipMemcpyHostToDevice, st));
+      }
+      std::vector<int64_t> again;
+      bool any_over = false;
+      for (int i = 0; i < nb; ++i) {
+        int64_t gid = run_gid[i];
+        if (bs[i] == 2) {
+          any_over = true;
+          drop[gid] = 1;
+          again.push_back(pending[i]);
+          continue;
+        }
+        for (int s = 0; s < NS; ++s) blk_pt2_all[(size_t)gid * NS + s] = bp[(size_t)i * NS + s];
+        blk_done[gid] = (bs[i] == 0) ? 1 : 2;
+        ncand_total += bc[i];
+        ngath_total += bg[i];
+        if (bs[i] == 0) ++nproc;
+      }
+      if (any_over) {
+        unsigned long long pn = (unsigned long long)pool_n;
+        QPX_HIP(hipMemcpyAsync(P.count.p, &pn, sizeof(pn), hipMemcpyHostToDevice, st));
+        compact(true);
+        for (auto gi : again) drop[bl_gid[gi]] = 0;
+        QPX_REQUIRE(pool_n < P.cap / 2 || n_keep >= P.cap / 4, "selection pool cannot be compacted");
+      } else if (pool_n > P.cap / 2) {
+        compact(false);
+      }
+      pending.swap(again);
+    }
+    if (pool_n > n_keep) compact(false);
+    g0 = g1 + 1;
+    // grow the batch: aim at ~ 32k blocks per launch
+    int per_gen = (int)std::max<int64_t>(1, (gen_first[std::min(g1, n_gen)] - gen_first[g1 - 1]));
+    batch_gens = std::max(1, std::min(4096, 32768 / per_gen));
+  }
+  // final: sort the pool by (key, determinant words) and keep n_keep
+  std::vector<double> keys(pool_n);
+  std::vector<uint64_t> dets((size_t)pool_n * 2 * NI);
+  QPX_HIP(hipMemcpyAsync(keys.data(), P.key.p, sizeof(double) * pool_n, hipMemcpyDeviceToHost, st));
+  QPX_HIP(hipMemcpyAsync(dets.data(), P.det.p, sizeof(uint64_t) * dets.size(), hipMemcpyDeviceToHost, st));
+  QPX_HIP(hipStreamSynchronize(st));
+  std::vector<int64_t> ord(pool_n);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
+    if (keys[a] != keys[b]) return keys[a] < keys[b];
+    const uint64_t *da = &dets[(size_t)a * 2 * NI], *db = &dets[(size_t)b * 2 * NI];
+    for (int w = 0; w < 2 * NI; ++w)
+      if (da[w] != db[w]) return da[w] < db[w];
+    return a < b;
+  });
+  int64_t nk = std::min<int64_t>(n_keep, pool_n);
+  for (int64_t i = 0; i < nk; ++i) {
+    out_vals[i] = keys[ord[i]];
+    std::memcpy(out_dets + (size_t)i * 2 * NI, &dets[(size_t)ord[i] * 2 * NI], sizeof(uint64_t) * 2 * NI);
+  }
+  *n_out = nk;
+  for (int s = 0; s < NS; ++s) {
+    double acc = 0.0;
+    for (int64_t b = 0; b < n_blocks_total; ++b) acc += blk_pt2_all[(size_t)b * NS + s];
+    pt2_out[s] = acc;
+  }
+  if (block_pt2_out) std::memcpy(block_pt2_out, blk_pt2_all.data(), sizeof(double) * blk_pt2_all.size());
+  if (block_info_out)
+    for (int64_t b = 0; b < n_blocks_total; ++b) {
+      const BlockDesc &d = all[b];
+      int64_t *r = block_info_out + 6 * b;
+      r[0] = d.gen; r[1] = d.s1; r[2] = d.h1; r[3] = d.s2; r[4] = d.h2; r[5] = blk_done[b];
+    }
+  c->sel_stats[0] = ncand_total;
+  c->sel_stats[1] = nproc;
+  c->sel_stats[2] = n_blocks_total;
+  c->sel_stats[3] = ngath_total;
+}
+
+}  // namespace
+
+static int select_impl(qpx_ctx *c, int n_states, const double *coef, int n_gen, int n_sel, const double *E0,
+                       int64_t n_keep, int subset, double *pt2, uint64_t *out_dets, double *out_vals, int64_t *n_out,
+                       const uint8_t *keep_mask, double *block_pt2, int64_t *block_info) {
+  return qpx_guard(c, [&] {
+    QPX_HIP(hipSetDevice(c->device));
+    QPX_REQUIRE(c->W.valid, "no wavefunction set");
+    QPX_REQUIRE(n_states >= 1 && n_states <= MAXNS, "n_states must be in 1..4");
+    QPX_REQUIRE(n_gen >= 1 && n_gen <= n_sel && n_sel <= c->W.n_det, "need 1 <= n_gen <= n_sel <= n_det");
+    QPX_REQUIRE(n_keep >= 0 && coef && E0 && pt2 && n_out, "bad selection arguments");
+    QPX_REQUIRE(subset >= 0, "subset must be >= 0");
+    switch (c->W.nint) {
+      case 1: run_selection<1>(c, n_states, coef, n_gen, n_sel, E0, n_keep, subset, pt2, out_dets, out_vals, n_out, keep_mask, block_pt2, block_info); break;
+      case 2: run_selection<2>(c, n_states, coef, n_gen, n_sel, E0, n_keep, subset, pt2, out_dets, out_vals, n_out, keep_mask, block_pt2, block_info); break;
+      case 3: run_selection<3>(c, n_states, coef, n_gen, n_sel, E0, n_keep, subset, pt2, out_dets, out_vals, n_out, keep_mask, block_pt2, block_info); break;
+      default: run_selection<4>(c, n_states, coef, n_gen, n_sel, E0, n_keep, subset, pt2, out_dets, out_vals, n_out, keep_mask, block_pt2, block_info); break;
+    }
+  });
+}
+
+extern "C" int qpx_select(qpx_ctx *c, int n_states, const double *coef, int n_gen, int n_sel, const double *E0,
+                          int64_t n_keep, int subset, double *pt2, uint64_t *out_dets, double *out_vals,
+                          int64_t *n_out) {
+  return select_impl(c, n_states, coef, n_gen, n_sel, E0, n_keep, subset, pt2, out_dets, out_vals, n_out, nullptr,
+                     nullptr, nullptr);
+}
+
+extern "C" int qpx_select_stats(qpx_ctx *c, int64_t *stats4) {
+  return qpx_guard(c, [&] {
+    for (int i = 0; i < 4; ++i) stats4[i] = c->sel_stats[i];
+  });
+}
+
+// Block-level variant: keep_mask (one byte per block in the reference's
+// enumeration order, all generators; NULL = all) restricts the enumerated
+// blocks; block_pt2 (n_blocks * n_states) and block_info (n_blocks * 6:
+// gen, s1, h1, s2, h2, status) receive per-block results (status 0 = not
+// run, 1 = evaluated, 2 = owned by an earlier generator).
+extern "C" int qpx_select_blocks(qpx_ctx *c, int n_states, const double *coef, int n_gen, int n_sel,
+                                 const double *E0, int64_t n_keep, const uint8_t *keep_mask, double *pt2,
+                                 uint64_t *out_dets, double *out_vals, int64_t *n_out, double *block_pt2,
+                                 int64_t *block_info) {
+  return select_impl(c, n_states, coef, n_gen, n_sel, E0, n_keep, 0, pt2, out_dets, out_vals, n_out, keep_mask,
+                     block_pt2, block_info);
+}
+
+// number of blocks of the first n_gen generators (for sizing block arrays)
+extern "C" int qpx_select_count_blocks(qpx_ctx *c, int n_gen, int64_t *n_blocks) {
+  return qpx_guard(c, [&] {
+    QPX_REQUIRE(c->W.valid, "no wavefunction set");
+    int na = c->H.na, nb = c->H.nb;
+    int64_t per = (int64_t)na * (na - 1) / 2 + (int64_t)nb * (nb - 1) / 2 + (int64_t)na * nb;
+    *n_blocks = per * n_gen;
+  });
+}

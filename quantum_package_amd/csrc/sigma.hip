@@ -1,0 +1,681 @@
+// sigma.hip -- v = H u, s = S^2 u over the determinant space (MI355X).
+//
+// Reference: H_S2_u_0_nstates_openmp{,_work} (src/Davidson/u0Hu0.irp.f:
+// 15-96,97-458).  The reference walks, for every determinant (a,b), (A) the
+// beta-singles b' of b and inside each column b' the alpha-singles of a,
+// and (B) the alpha singles/doubles inside column b and the beta
+// singles/doubles inside row a, plus the diagonal; rows are owned by one
+// thread (no write conflicts).  The same ownership is kept here, with two
+// device strategies:
+//
+//  * dense-map path (unique alpha/beta sets small enough that one column of
+//    u fits in LDS): one workgroup owns a beta column b.  For every
+//    beta-single b' of b it stages column b' of u into LDS as a dense
+//    alpha-indexed map (absent determinants read as zero) together with the
+//    integral slice (h_a p_a | h_b p_b) for the beta excitation, and every
+//    thread gathers its alpha-singles from LDS.  The same-spin alpha part
+//    reads column b itself; the same-spin beta part runs as a second kernel
+//    over alpha rows with a beta-indexed map.  u is read from HBM once per
+//    staged column instead of once per connected pair.
+//  * generic path (any size): one wavefront owns a determinant and scans,
+//    with coalesced lane-strided loads, the column b and row a for
+//    same-spin partners and, for the alpha/beta doubles, either the rows of
+//    the alpha-singles of a or the columns of the beta-singles of b,
+//    whichever holds fewer entries (the reference scans the columns).
+//    Hits are reduced across the wavefront at the end.
+//
+// Matrix elements: alpha/beta double = phase_a phase_b (h_a p_a|h_b p_b);
+// same-spin double = phase ((h1 p1|h2 p2) - (h1 p2|h2 p1)); single h->p =
+// phase (h_hp + sum_{k in same-spin string} ((hp|kk)-(hk|kp)) + sum_{k in
+// other-spin string} (hp|kk)); S^2 off-diagonal = -phase for the alpha/beta
+// spin exchange (src/Determinants/s2.irp.f:32-63).
+#include <algorithm>
+#include "qpx_internal.h"
+#include "qpx_device.h"
+
+using namespace qpx;
+
+template <typename T>
+struct Tab;
+template <>
+struct Tab<double> {
+  static const double *V(const Ham &H) { return H.V.p; }
+  static const double *sval(const SpinConn &c) { return c.s_val.p; }
+  static const double *dval(const SpinConn &c) { return c.d_val.p; }
+  static const double *hcore(const Ham &H) { return H.hcore.p; }
+};
+template <>
+struct Tab<float> {
+  static const float *V(const Ham &H) { return H.Vf.p; }
+  static const float *sval(const SpinConn &c) { return c.s_valf.p; }
+  static const float *dval(const SpinConn &c) { return c.d_valf.p; }
+  static const float *hcore(const Ham &H) { return H.hcoref.p; }
+};
+
+// ---------------------------------------------------------------------------
+// dense-map path
+// ---------------------------------------------------------------------------
+struct DenseArgs {
+  const uint64_t *astr, *bstr;
+  const int *col_ptr, *csc_a, *csc_b, *row_ptr, *csr_b, *csr2csc;
+  const int *as_ptr, *as_idx; const uint32_t *as_info;  // alpha singles
+  const int *ad_ptr, *ad_idx;                           // alpha doubles
+  const int *bs_ptr, *bs_idx; const uint32_t *bs_info;  // beta singles
+  const int *bd_ptr, *bd_idx;
+  const double *hdiag, *sdiag;
+  int n_a, n_b, M, npair;
+};
+
+// LDS layout: map[n_map * NST] | slice[M*M]
+template <typename T, int NI, int NST, int DPT>
+__global__ __launch_bounds__(1024) void sigma_dense_col_kernel(DenseArgs A, const T *__restrict__ V,
+                                                               const T *__restrict__ as_val,
+                                                               const T *__restrict__ ad_val,
+                                                               const T *__restrict__ u, T *__restrict__ v,
+                                                               T *__restrict__ s) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *map = reinterpret_cast<T *>(lds_raw);
+  T *slice = map + (size_t)A.n_a * NST;
+  const int b = blockIdx.x;
+  const int c0 = A.col_ptr[b], c1 = A.col_ptr[b + 1];
+  const int n_col = c1 - c0;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int M = A.M, M2 = M * M;
+
+  int my_a[DPT];
+  T acc[DPT][NST], accs[DPT][NST];
+#pragma unroll
+  for (int d = 0; d < DPT; ++d) {
+    int e = tid + d * nth;
+    my_a[d] = (e < n_col) ? A.csc_a[c0 + e] : -1;
+#pragma unroll
+    for (int st = 0; st < NST; ++st) acc[d][st] = accs[d][st] = T(0);
+  }
+  uint64_t bw[NI];
+#pragma unroll
+  for (int w = 0; w < NI; ++w) bw[w] = A.bstr[(int64_t)b * NI + w];
+
+  // ---- (A) alpha/beta doubles: beta-singles b' of b ----
+  for (int eb = A.bs_ptr[b]; eb < A.bs_ptr[b + 1]; ++eb) {
+    const int bp = A.bs_idx[eb];
+    const uint32_t binfo = A.bs_info[eb];
+    const int hb = binfo & 0xff, pb = (binfo >> 8) & 0xff, phb = (binfo >> 16) & 1;
+    const int l0 = A.col_ptr[bp], l1 = A.col_ptr[bp + 1];
+    __syncthreads();
+    if (l1 - l0 != A.n_a) {
+      for (int x = tid; x < A.n_a * NST; x += nth) map[x] = T(0);
+      __syncthreads();
+    }
+    for (int e = l0 + tid; e < l1; e += nth) {
+      int ai = A.csc_a[e];
+#pragma unroll
+      for (int st = 0; st < NST; ++st) map[ai * NST + st] = u[(int64_t)e * NST + st];
+    }
+    const int64_t Pb = pair_index(hb, pb);
+    for (int x = tid; x < M2; x += nth) {
+      int h = x / M, p = x - h * M;
+      slice[x] = V[pair_index(h, p) * A.npair + Pb];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < DPT; ++d) {
+      const int a = my_a[d];
+      if (a < 0) continue;
+      for (int es = A.as_ptr[a]; es < A.as_ptr[a + 1]; ++es) {
+        const int ap = A.as_idx[es];
+        const uint32_t ai = A.as_info[es];
+        const int ha = ai & 0xff, pa = (ai >> 8) & 0xff;
+        T val = slice[ha * M + pa];
+        const bool neg = ((ai >> 16) & 1) ^ phb;
+        if (neg) val = -val;
+        const bool flip = (ha == pb) && (pa == hb);
+#pragma unroll
+        for (int st = 0; st < NST; ++st) {
+          T uu = map[ap * NST + st];
+          acc[d][st] += val * uu;
+          if (flip) accs[d][st] += neg ? uu : -uu;
+        }
+      }
+    }
+  }
+
+  // ---- (B) same-spin alpha inside column b, and the diagonal ----
+  __syncthreads();
+  if (n_col != A.n_a) {
+    for (int x = tid; x < A.n_a * NST; x += nth) map[x] = T(0);
+    __syncthreads();
+  }
+  for (int e = c0 + tid; e < c1; e += nth) {
+    int ai = A.csc_a[e];
+#pragma unroll
+    for (int st = 0; st < NST; ++st) map[ai * NST + st] = u[(int64_t)e * NST + st];
+  }
+  // g_b(h,p) = sum_{k in b} (hp|kk)
+  for (int x = tid; x < M2; x += nth) {
+    int h = x / M, p = x - h * M;
+    const int64_t P = pair_index(h, p);
+    T g = T(0);
+#pragma unroll
+    for (int w = 0; w < NI; ++w) {
+      uint64_t y = bw[w];
+      while (y) {
+        int k = 64 * w + __ffsll((unsigned long long)y) - 1;
+        y &= y - 1;
+        g += V[P * A.npair + pair_index(k, k)];
+      }
+    }
+    slice[x] = g;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < DPT; ++d) {
+    const int a = my_a[d];
+    if (a < 0) continue;
+    for (int es = A.as_ptr[a]; es < A.as_ptr[a + 1]; ++es) {
+      const int ap = A.as_idx[es];
+      const uint32_t ai = A.as_info[es];
+      const int ha = ai & 0xff, pa = (ai >> 8) & 0xff;
+      T g = slice[ha * M + pa];
+      T val = as_val[es] + (((ai >> 16) & 1) ? -g : g);
+#pragma unroll
+      for (int st = 0; st < NST; ++st) acc[d][st] += val * map[ap * NST + st];
+    }
+    for (int ed = A.ad_ptr[a]; ed < A.ad_ptr[a + 1]; ++ed) {
+      const int ap = A.ad_idx[ed];
+      T val = ad_val[ed];
+#pragma unroll
+      for (int st = 0; st < NST; ++st) acc[d][st] += val * map[ap * NST + st];
+    }
+    const int e = c0 + tid + d * nth;
+    const T hd = (T)A.hdiag[e], sd = (T)A.sdiag[e];
+#pragma unroll
+    for (int st = 0; st < NST; ++st) {
+      T uu = map[a * NST + st];
+      v[(int64_t)e * NST + st] = acc[d][st] + hd * uu;
+      s[(int64_t)e * NST + st] = accs[d][st] + sd * uu;
+    }
+  }
+}
+
+// same-spin beta part: one workgroup per alpha row a
+template <typename T, int NI, int NST, int DPT>
+__global__ __launch_bounds__(1024) void sigma_dense_row_kernel(DenseArgs A, const T *__restrict__ V,
+                                                               const T *__restrict__ bs_val,
+                                                               const T *__restrict__ bd_val,
+                                                               const T *__restrict__ u, T *__restrict__ v) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *map = reinterpret_cast<T *>(lds_raw);
+  T *slice = map + (size_t)A.n_b * NST;
+  const int a = blockIdx.x;
+  const int r0 = A.row_ptr[a], r1 = A.row_ptr[a + 1];
+  const int n_row = r1 - r0;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int M = A.M, M2 = M * M;
+  if (n_row != A.n_b) {
+    for (int x = tid; x < A.n_b * NST; x += nth) map[x] = T(0);
+    __syncthreads();
+  }
+  for (int t = r0 + tid; t < r1; t += nth) {
+    int bi = A.csr_b[t];
+    int e = A.csr2csc[t];
+#pragma unroll
+    for (int st = 0; st < NST; ++st) map[bi * NST + st] = u[(int64_t)e * NST + st];
+  }
+  uint64_t aw[NI];
+#pragma unroll
+  for (int w = 0; w < NI; ++w) aw[w] = A.astr[(int64_t)a * NI + w];
+  for (int x = tid; x < M2; x += nth) {
+    int h = x / M, p = x - h * M;
+    const int64_t P = pair_index(h, p);
+    T g = T(0);
+#pragma unroll
+    for (int w = 0; w < NI; ++w) {
+      uint64_t y = aw[w];
+      while (y) {
+        int k = 64 * w + __ffsll((unsigned long long)y) - 1;
+        y &= y - 1;
+        g += V[P * A.npair + pair_index(k, k)];
+      }
+    }
+    slice[x] = g;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < DPT; ++d) {
+    const int t = r0 + tid + d * nth;
+    if (t >= r1) continue;
+    const int b = A.csr_b[t];
+    T acc[NST];
+#pragma unroll
+    for (int st = 0; st < NST; ++st) acc[st] = T(0);
+    for (int es = A.bs_ptr[b]; es < A.bs_ptr[b + 1]; ++es) {
+      const int bp = A.bs_idx[es];
+      const uint32_t bi = A.bs_info[es];
+      const int hb = bi & 0xff, pb = (bi >> 8) & 0xff;
+      T g = slice[hb * M + pb];
+      T val = bs_val[es] + (((bi >> 16) & 1) ? -g : g);
+#pragma unroll
+      for (int st = 0; st < NST; ++st) acc[st] += val * map[bp * NST + st];
+    }
+    for (int ed = A.bd_ptr[b]; ed < A.bd_ptr[b + 1]; ++ed) {
+      const int bp = A.bd_idx[ed];
+      T val = bd_val[ed];
+#pragma unroll
+      for (int st = 0; st < NST; ++st) acc[st] += val * map[bp * NST + st];
+    }
+    const int e = A.csr2csc[t];
+#pragma unroll
+    for (int st = 0; st < NST; ++st) v[(int64_t)e * NST + st] += acc[st];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// generic path: one wavefront per determinant
+// ---------------------------------------------------------------------------
+struct GenArgs {
+  const uint64_t *astr, *bstr;
+  const int *col_ptr, *csc_a, *csc_b, *row_ptr, *csr_b, *csr2csc;
+  const int *as_ptr, *as_idx; const uint32_t *as_info;
+  const int *bs_ptr, *bs_idx; const uint32_t *bs_info;
+  const int64_t *cost_a, *cost_b;
+  const double *hdiag, *sdiag;
+  int64_t n_det;
+  int M, npair, nst;
+};
+
+constexpr int GMAXST = 8;
+
+template <typename T, int NI>
+__device__ inline T single_elem(const uint64_t *same, const uint64_t *other, int h, int p, const T *V,
+                                const T *hcore, int M, int npair) {
+  const int64_t P = pair_index(h, p);
+  T e = hcore[h * M + p];
+#pragma unroll
+  for (int w = 0; w < NI; ++w) {
+    uint64_t y = same[w];
+    while (y) {
+      int k = 64 * w + __ffsll((unsigned long long)y) - 1;
+      y &= y - 1;
+      e += V[P * npair + pair_index(k, k)] - V[pair_index(h, k) * npair + pair_index(k, p)];
+    }
+    y = other[w];
+    while (y) {
+      int k = 64 * w + __ffsll((unsigned long long)y) - 1;
+      y &= y - 1;
+      e += V[P * npair + pair_index(k, k)];
+    }
+  }
+  return e;
+}
+
+template <typename T, int NI>
+__global__ __launch_bounds__(256) void sigma_generic_kernel(GenArgs A, const T *__restrict__ V,
+                                                            const T *__restrict__ hcore,
+                                                            const T *__restrict__ u, T *__restrict__ v,
+                                                            T *__restrict__ s) {
+  const int lane = threadIdx.x & 63;
+  const int64_t k = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (k >= A.n_det) return;
+  const int nst = A.nst;
+  const int M = A.M, npair = A.npair;
+  const int a = A.csc_a[k], b = A.csc_b[k];
+  uint64_t aw[NI], bw[NI];
+#pragma unroll
+  for (int w = 0; w < NI; ++w) {
+    aw[w] = A.astr[(int64_t)a * NI + w];
+    bw[w] = A.bstr[(int64_t)b * NI + w];
+  }
+  T acc[GMAXST], accs[GMAXST];
+#pragma unroll
+  for (int st = 0; st < GMAXST; ++st) acc[st] = accs[st] = T(0);
+
+  // same-spin alpha: scan column b
+  for (int e = A.col_ptr[b] + lane; e < A.col_ptr[b + 1]; e += 64) {
+    const int ai = A.csc_a[e];
+    if (ai == a) continue;
+    uint64_t xw[NI];
+    int d = 0;
+#pragma unroll
+    for (int w = 0; w < NI; ++w) {
+      xw[w] = A.astr[(int64_t)ai * NI + w];
+      d += __popcll(aw[w] ^ xw[w]);
+    }
+    T hij;
+    if (d == 2) {
+      int h, p;
+      single_hp<NI>(aw, xw, &h, &p);
+      hij = single_elem<T, NI>(aw, bw, h, p, V, hcore, M, npair);
+      if (between_parity<NI>(aw, h, p)) hij = -hij;
+    } else if (d == 4) {
+      int h1, h2, p1, p2;
+      double_hp<NI>(aw, xw, &h1, &h2, &p1, &p2);
+      hij = V[pair_index(h1, p1) * npair + pair_index(h2, p2)] - V[pair_index(h1, p2) * npair + pair_index(h2, p1)];
+      if (double_parity<NI>(aw, h1, h2, p1, p2)) hij = -hij;
+    } else {
+      continue;
+    }
+#pragma unroll
+    for (int st = 0; st < GMAXST; ++st)
+      if (st < nst) acc[st] += hij * u[(int64_t)e * nst + st];
+  }
+  // same-spin beta: scan row a
+  for (int t = A.row_ptr[a] + lane; t < A.row_ptr[a + 1]; t += 64) {
+    const int bi = A.csr_b[t];
+    if (bi == b) continue;
+    uint64_t xw[NI];
+    int d = 0;
+#pragma unroll
+    for (int w = 0; w < NI; ++w) {
+      xw[w] = A.bstr[(int64_t)bi * NI + w];
+      d += __popcll(bw[w] ^ xw[w]);
+    }
+    T hij;
+    if (d == 2) {
+      int h, p;
+      single_hp<NI>(bw, xw, &h, &p);
+      hij = single_elem<T, NI>(bw, aw, h, p, V, hcore, M, npair);
+      if (between_parity<NI>(bw, h, p)) hij = -hij;
+    } else if (d == 4) {
+      int h1, h2, p1, p2;
+      double_hp<NI>(bw, xw, &h1, &h2, &p1, &p2);
+      hij = V[pair_index(h1, p1) * npair + pair_index(h2, p2)] - V[pair_index(h1, p2) * npair + pair_index(h2, p1)];
+      if (double_parity<NI>(bw, h1, h2, p1, p2)) hij = -hij;
+    } else {
+      continue;
+    }
+    const int64_t e = A.csr2csc[t];
+#pragma unroll
+    for (int st = 0; st < GMAXST; ++st)
+      if (st < nst) acc[st] += hij * u[e * nst + st];
+  }
+  // alpha/beta doubles
+  if (A.cost_a[a] <= A.cost_b[b]) {
+    for (int es = A.as_ptr[a]; es < A.as_ptr[a + 1]; ++es) {
+      const int ap = A.as_idx[es];
+      const uint32_t ai = A.as_info[es];
+      const int ha = ai & 0xff, pa = (ai >> 8) & 0xff, pha = (ai >> 16) & 1;
+      const int64_t Pa = pair_index(ha, pa);
+      for (int t = A.row_ptr[ap] + lane; t < A.row_ptr[ap + 1]; t += 64) {
+        const int bi = A.csr_b[t];
+        uint64_t xw[NI];
+        int d = 0;
+#pragma unroll
+        for (int w = 0; w < NI; ++w) {
+          xw[w] = A.bstr[(int64_t)bi * NI + w];
+          d += __popcll(bw[w] ^ xw[w]);
+        }
+        if (d != 2) continue;
+        int hb, pb;
+        single_hp<NI>(bw, xw, &hb, &pb);
+        const bool neg = pha ^ between_parity<NI>(bw, hb, pb);
+        T hij = V[Pa * npair + pair_index(hb, pb)];
+        if (neg) hij = -hij;
+        const bool flip = (ha == pb) && (pa == hb);
+        const int64_t e = A.csr2csc[t];
+#pragma unroll
+        for (int st = 0; st < GMAXST; ++st)
+          if (st < nst) {
+            T uu = u[e * nst + st];
+            acc[st] += hij * uu;
+            if (flip) accs[st] += neg ? uu : -uu;
+          }
+      }
+    }
+  } else {
+    for (int es = A.bs_ptr[b]; es < A.bs_ptr[b + 1]; ++es) {
+      const int bp = A.bs_idx[es];
+      const uint32_t bi = A.bs_info[es];
+      const int hb = bi & 0xff, pb = (bi >> 8) & 0xff, phb = (bi >> 16) & 1;
+      const int64_t Pb = pair_index(hb, pb);
+      for (int e = A.col_ptr[bp] + lane; e < A.col_ptr[bp + 1]; e += 64) {
+        const int ai = A.csc_a[e];
+        uint64_t xw[NI];
+        int d = 0;
+#pragma unroll
+        for (int w = 0; w < NI; ++w) {
+          xw[w] = A.astr[(int64_t)ai * NI + w];
+          d += __popcll(aw[w] ^ xw[w]);
+        }
+        if (d != 2) continue;
+        int ha, pa;
+        single_hp<NI>(aw, xw, &ha, &pa);
+        const bool neg = phb ^ between_parity<NI>(aw, ha, pa);
+        T hij = V[pair_index(ha, pa) * npair + Pb];
+        if (neg) hij = -hij;
+        const bool flip = (ha == pb) && (pa == hb);
+#pragma unroll
+        for (int st = 0; st < GMAXST; ++st)
+          if (st < nst) {
+            T uu = u[(int64_t)e * nst + st];
+            acc[st] += hij * uu;
+            if (flip) accs[st] += neg ? uu : -uu;
+          }
+      }
+    }
+  }
+  // wavefront reduction
+#pragma unroll
+  for (int st = 0; st < GMAXST; ++st) {
+    if (st >= nst) break;
+    T x = acc[st], y = accs[st];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      x += __shfl_xor(x, off, 64);
+      y += __shfl_xor(y, off, 64);
+    }
+    acc[st] = x;
+    accs[st] = y;
+  }
+  if (lane == 0) {
+    const T hd = (T)A.hdiag[k], sd = (T)A.sdiag[k];
+#pragma unroll
+    for (int st = 0; st < GMAXST; ++st)
+      if (st < nst) {
+        T uu = u[k * nst + st];
+        v[k * nst + st] = acc[st] + hd * uu;
+        s[k * nst + st] = accs[st] + sd * uu;
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// dispatch
+// ---------------------------------------------------------------------------
+static DenseArgs dense_args(qpx_ctx *c) {
+  Wfn &W = c->W;
+  DenseArgs A;
+  A.astr = W.astr.p; A.bstr = W.bstr.p;
+  A.col_ptr = W.col_ptr.p; A.csc_a = W.csc_a.p; A.csc_b = W.csc_b.p;
+  A.row_ptr = W.row_ptr.p; A.csr_b = W.csr_b.p; A.csr2csc = W.csr2csc.p;
+  A.as_ptr = W.ca.s_ptr.p; A.as_idx = W.ca.s_idx.p; A.as_info = W.ca.s_info.p;
+  A.ad_ptr = W.ca.d_ptr.p; A.ad_idx = W.ca.d_idx.p;
+  A.bs_ptr = W.cb.s_ptr.p; A.bs_idx = W.cb.s_idx.p; A.bs_info = W.cb.s_info.p;
+  A.bd_ptr = W.cb.d_ptr.p; A.bd_idx = W.cb.d_idx.p;
+  A.hdiag = W.hdiag.p; A.sdiag = W.sdiag.p;
+  A.n_a = W.n_a; A.n_b = W.n_b; A.M = c->H.M; A.npair = c->H.npair;
+  return A;
+}
+
+template <typename T, int NI, int NST, int DPT>
+static void launch_dense(qpx_ctx *c, const T *u, T *v, T *s, hipStream_t st) {
+  DenseArgs A = dense_args(c);
+  const Ham &H = c->H;
+  const Wfn &W = c->W;
+  size_t lds_col = ((size_t)W.n_a * NST + (size_t)H.M * H.M) * sizeof(T);
+  size_t lds_row = ((size_t)W.n_b * NST + (size_t)H.M * H.M) * sizeof(T);
+  auto kc = sigma_dense_col_kernel<T, NI, NST, DPT>;
+  auto kr = sigma_dense_row_kernel<T, NI, NST, DPT>;
+  QPX_HIP(hipFuncSetAttribute((const void *)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_col));
+  QPX_HIP(hipFuncSetAttribute((const void *)kr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_row));
+  hipLaunchKernelGGL(kc, dim3(W.n_b), dim3(1024), lds_col, st, A, Tab<T>::V(H), Tab<T>::sval(W.ca),
+                     Tab<T>::dval(W.ca), u, v, s);
+  QPX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(kr, dim3(W.n_a), dim3(1024), lds_row, st, A, Tab<T>::V(H), Tab<T>::sval(W.cb),
+                     Tab<T>::dval(W.cb), u, v);
+  QPX_HIP(hipGetLastError());
+}
+
+template <typename T, int NI, int NST>
+static void launch_dense_dpt(qpx_ctx *c, int dpt, const T *u, T *v, T *s, hipStream_t st) {
+  switch (dpt) {
+    case 1: launch_dense<T, NI, NST, 1>(c, u, v, s, st); break;
+    case 2: launch_dense<T, NI, NST, 2>(c, u, v, s, st); break;
+    case 3: launch_dense<T, NI, NST, 3>(c, u, v, s, st); break;
+    default: launch_dense<T, NI, NST, 4>(c, u, v, s, st); break;
+  }
+}
+
+template <typename T, int NI>
+static void launch_dense_nst(qpx_ctx *c, int nst, int dpt, const T *u, T *v, T *s, hipStream_t st) {
+  switch (nst) {
+    case 1: launch_dense_dpt<T, NI, 1>(c, dpt, u, v, s, st); break;
+    case 2: launch_dense_dpt<T, NI, 2>(c, dpt, u, v, s, st); break;
+    case 3: launch_dense_dpt<T, NI, 3>(c, dpt, u, v, s, st); break;
+    default: launch_dense_dpt<T, NI, 4>(c, dpt, u, v, s, st); break;
+  }
+}
+
+template <typename T>
+static void launch_generic(qpx_ctx *c, int nst, const T *u, T *v, T *s, hipStream_t st) {
+  Wfn &W = c->W;
+  GenArgs A;
+  A.astr = W.astr.p; A.bstr = W.bstr.p;
+  A.col_ptr = W.col_ptr.p; A.csc_a = W.csc_a.p; A.csc_b = W.csc_b.p;
+  A.row_ptr = W.row_ptr.p; A.csr_b = W.csr_b.p; A.csr2csc = W.csr2csc.p;
+  A.as_ptr = W.ca.s_ptr.p; A.as_idx = W.ca.s_idx.p; A.as_info = W.ca.s_info.p;
+  A.bs_ptr = W.cb.s_ptr.p; A.bs_idx = W.cb.s_idx.p; A.bs_info = W.cb.s_info.p;
+  A.cost_a = W.cost_a.p; A.cost_b = W.cost_b.p;
+  A.hdiag = W.hdiag.p; A.sdiag = W.sdiag.p;
+  A.n_det = W.n_det; A.M = c->H.M; A.npair = c->H.npair; A.nst = nst;
+  const int wpb = 4;
+  int64_t blocks = (W.n_det + wpb - 1) / wpb;
+  QPX_REQUIRE(blocks < (1LL << 31), "too many determinants for one launch");
+  const Ham &H = c->H;
+  switch (W.nint) {
+    case 1: hipLaunchKernelGGL((sigma_generic_kernel<T, 1>), dim3((unsigned)blocks), dim3(64 * wpb), 0, st, A, Tab<T>::V(H), Tab<T>::hcore(H), u, v, s); break;
+    case 2: hipLaunchKernelGGL((sigma_generic_kernel<T, 2>), dim3((unsigned)blocks), dim3(64 * wpb), 0, st, A, Tab<T>::V(H), Tab<T>::hcore(H), u, v, s); break;
+    case 3: hipLaunchKernelGGL((sigma_generic_kernel<T, 3>), dim3((unsigned)blocks), dim3(64 * wpb), 0, st, A, Tab<T>::V(H), Tab<T>::hcore(H), u, v, s); break;
+    default: hipLaunchKernelGGL((sigma_generic_kernel<T, 4>), dim3((unsigned)blocks), dim3(64 * wpb), 0, st, A, Tab<T>::V(H), Tab<T>::hcore(H), u, v, s); break;
+  }
+  QPX_HIP(hipGetLastError());
+}
+
+namespace qpx {
+
+// dense-map path is usable when one column (row) of u plus the M x M slice
+// fits in the LDS of one workgroup and columns (rows) hold <= 4096 entries
+static bool dense_ok(qpx_ctx *c, int nst, size_t tsize) {
+  const Wfn &W = c->W;
+  const Ham &H = c->H;
+  if (W.nint > 2 || nst > 4) return false;
+  if (W.max_col > 4096 || W.max_row > 4096) return false;
+  size_t need_c = ((size_t)W.n_a * nst + (size_t)H.M * H.M) * tsize;
+  size_t need_r = ((size_t)W.n_b * nst + (size_t)H.M * H.M) * tsize;
+  size_t lim = (size_t)c->lds_per_block;
+  return need_c <= lim && need_r <= lim;
+}
+
+void sigma_device(qpx_ctx *c, int prec, int nst, const void *u, void *v, void *s, hipStream_t st) {
+  QPX_REQUIRE(c->W.valid, "no wavefunction set");
+  QPX_REQUIRE(nst >= 1 && nst <= GMAXST, "n_st must be in 1..8");
+  QPX_REQUIRE(prec == QPX_FP64 || prec == QPX_FP32, "prec must be QPX_FP64 or QPX_FP32");
+  size_t ts = prec == QPX_FP64 ? sizeof(double) : sizeof(float);
+  bool dense = dense_ok(c, nst, ts);
+  if (c->force_path == 2) dense = false;
+  if (c->force_path == 1) QPX_REQUIRE(dense, "dense-map sigma path not applicable to this wavefunction");
+  const Wfn &W = c->W;
+  int dpt = std::max((W.max_col + 1023) / 1024, (W.max_row + 1023) / 1024);
+  if (dpt < 1) dpt = 1;
+  if (dense) {
+    c->sigma_path = 1;
+    if (prec == QPX_FP64) {
+      if (W.nint == 1) launch_dense_nst<double, 1>(c, nst, dpt, (const double *)u, (double *)v, (double *)s, st);
+      else launch_dense_nst<double, 2>(c, nst, dpt, (const double *)u, (double *)v, (double *)s, st);
+    } else {
+      if (W.nint == 1) launch_dense_nst<float, 1>(c, nst, dpt, (const float *)u, (float *)v, (float *)s, st);
+      else launch_dense_nst<float, 2>(c, nst, dpt, (const float *)u, (float *)v, (float *)s, st);
+    }
+  } else {
+    c->sigma_path = 2;
+    if (prec == QPX_FP64) launch_generic<double>(c, nst, (const double *)u, (double *)v, (double *)s, st);
+    else launch_generic<float>(c, nst, (const float *)u, (float *)v, (float *)s, st);
+  }
+}
+
+}  // namespace qpx
+
+// ---------------------------------------------------------------------------
+// host-facing entry points
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void gather_u_kernel(const double *__restrict__ u, int64_t sze, const int64_t *__restrict__ perm,
+                                int64_t n, int nst, T *__restrict__ ut) {
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n * nst; x += (int64_t)gridDim.x * blockDim.x) {
+    int64_t k = x / nst;
+    int st = (int)(x - k * nst);
+    ut[x] = (T)u[perm[k] + sze * st];
+  }
+}
+
+template <typename T>
+__global__ void scatter_v_kernel(const T *__restrict__ vt, const int64_t *__restrict__ perm, int64_t n, int nst,
+                                 int64_t sze, double *__restrict__ v) {
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n * nst; x += (int64_t)gridDim.x * blockDim.x) {
+    int64_t k = x / nst;
+    int st = (int)(x - k * nst);
+    v[perm[k] + sze * st] = (double)vt[x];
+  }
+}
+
+template <typename T>
+static void sigma_host(qpx_ctx *c, int prec, int nst, int64_t sze, const double *u, double *v, double *s) {
+  Wfn &W = c->W;
+  const int64_t n = W.n_det;
+  hipStream_t st = c->stream;
+  char *io = c->ws_sigma_io.get(sizeof(double) * (size_t)sze * nst * 3);
+  double *du = (double *)io, *dv = du + (size_t)sze * nst, *ds = dv + (size_t)sze * nst;
+  char *tb = c->ws_sigma_t.get(sizeof(T) * (size_t)n * nst * 3);
+  T *ut = (T *)tb, *vt = ut + (size_t)n * nst, *stt = vt + (size_t)n * nst;
+  QPX_HIP(hipMemcpyAsync(du, u, sizeof(double) * (size_t)sze * nst, hipMemcpyHostToDevice, st));
+  int grid = (int)std::min<int64_t>((n * nst + 255) / 256, 65536);
+  hipLaunchKernelGGL(gather_u_kernel<T>, dim3(grid), dim3(256), 0, st, du, sze, W.perm_d.p, n, nst, ut);
+  QPX_HIP(hipGetLastError());
+  sigma_device(c, prec, nst, ut, vt, stt, st);
+  QPX_HIP(hipMemsetAsync(dv, 0, sizeof(double) * (size_t)sze * nst * 2, st));
+  hipLaunchKernelGGL(scatter_v_kernel<T>, dim3(grid), dim3(256), 0, st, vt, W.perm_d.p, n, nst, sze, dv);
+  hipLaunchKernelGGL(scatter_v_kernel<T>, dim3(grid), dim3(256), 0, st, stt, W.perm_d.p, n, nst, sze, ds);
+  QPX_HIP(hipGetLastError());
+  QPX_HIP(hipMemcpyAsync(v, dv, sizeof(double) * (size_t)sze * nst, hipMemcpyDeviceToHost, st));
+  QPX_HIP(hipMemcpyAsync(s, ds, sizeof(double) * (size_t)sze * nst, hipMemcpyDeviceToHost, st));
+  QPX_HIP(hipStreamSynchronize(st));
+}
+
+extern "C" int qpx_sigma(qpx_ctx *c, int prec, int nst, int64_t sze, const double *u, double *v, double *s) {
+  return qpx_guard(c, [&] {
+    QPX_HIP(hipSetDevice(c->device));
+    QPX_REQUIRE(c->W.valid, "no wavefunction set");
+    QPX_REQUIRE(sze >= c->W.n_det, "sze must be >= n_det");
+    QPX_REQUIRE(u && v && s, "null vector");
+    if (prec == QPX_FP64) sigma_host<double>(c, prec, nst, sze, u, v, s);
+    else sigma_host<float>(c, prec, nst, sze, u, v, s);
+  });
+}
+
+extern "C" int qpx_sigma_device(qpx_ctx *c, int prec, int nst, const void *u_t, void *v_t, void *s_t,
+                                void *stream) {
+  return qpx_guard(c, [&] {
+    QPX_HIP(hipSetDevice(c->device));
+    QPX_REQUIRE(u_t && v_t && s_t, "null device vector");
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    sigma_device(c, prec, nst, u_t, v_t, s_t, st);
+  });
+}
+
+// test hook: 0 = automatic, 1 = dense-map only, 2 = generic only
+extern "C" int qpx_debug_force_sigma_path(qpx_ctx *c, int path, int *last_path) {
+  return qpx_guard(c, [&] {
+    QPX_REQUIRE(path >= 0 && path <= 2, "path must be 0, 1 or 2");
+    c->force_path = path;
+    if (last_path) *last_path = c->sigma_path;
+  });
+}
