@@ -663,12 +663,14 @@ static void select_singles_and_doubles(selw *w, int i_generator, const bk_t *hol
             if (monoAdo) { BORB(w, h2, s2) = 0; monoAdo = 0; }
           }
           maskInd += 1;
+#pragma omp atomic
           S->n_blocks_total++;
           int keep = (subset == 0 || (maskInd % S->fragment_count) == (subset - 1));
           if (keep && S->block_keep)
             keep = S->block_keep[S->block_keep_offsets[i_generator - 1] + maskInd] != 0;
           if (keep) {
             if (spot_isinwf(w, mask, fullmini, fullint, nfull, i_generator)) continue;
+#pragma omp atomic
             S->n_blocks++;
             splash_pq(w, mask, sp, minilist, interesting, nint_, i_generator);
             double blk_e = 0.0;

@@ -35,6 +35,11 @@ SIGNATURES = [
 # internal hooks used by the tests (not part of the C-ABI header)
 EXTRA = [
     ("qpx_debug_force_sigma_path", C.c_int, [c_vp, C.c_int, c_ip]),
+    ("qpx_debug_sigma_kernel_times", C.c_int, [c_vp, C.c_int, C.c_int, c_vp, c_vp, c_vp, c_vp,
+                                               C.POINTER(C.c_float), C.c_int]),
+    ("qpx_select_blocks", C.c_int, [c_vp, C.c_int, c_dp, C.c_int, C.c_int, c_dp, C.c_int64,
+                                    C.POINTER(C.c_uint8), c_dp, c_u64p, c_dp, c_i64p, c_dp, c_i64p]),
+    ("qpx_select_count_blocks", C.c_int, [c_vp, C.c_int, c_i64p]),
 ]
 
 _LIB = None

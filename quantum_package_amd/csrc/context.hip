@@ -63,6 +63,8 @@ extern "C" int qpx_destroy(qpx_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     (void)hipStreamDestroy(c->stream);
   }
+  for (int i = 0; i < 3; ++i)
+    if (c->kev[i]) (void)hipEventDestroy(c->kev[i]);
   delete c;  // DBuf destructors free device memory
   return 0;
 }

@@ -507,12 +507,15 @@ static void launch_dense(qpx_ctx *c, const T *u, T *v, T *s, hipStream_t st) {
   auto kr = sigma_dense_row_kernel<T, NI, NST, DPT>;
   QPX_HIP(hipFuncSetAttribute((const void *)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_col));
   QPX_HIP(hipFuncSetAttribute((const void *)kr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_row));
+  if (c->ktiming) QPX_HIP(hipEventRecord(c->kev[0], st));
   hipLaunchKernelGGL(kc, dim3(W.n_b), dim3(1024), lds_col, st, A, Tab<T>::V(H), Tab<T>::sval(W.ca),
                      Tab<T>::dval(W.ca), u, v, s);
   QPX_HIP(hipGetLastError());
+  if (c->ktiming) QPX_HIP(hipEventRecord(c->kev[1], st));
   hipLaunchKernelGGL(kr, dim3(W.n_a), dim3(1024), lds_row, st, A, Tab<T>::V(H), Tab<T>::sval(W.cb),
                      Tab<T>::dval(W.cb), u, v);
   QPX_HIP(hipGetLastError());
+  if (c->ktiming) QPX_HIP(hipEventRecord(c->kev[2], st));
 }
 
 template <typename T, int NI, int NST>
@@ -551,6 +554,7 @@ static void launch_generic(qpx_ctx *c, int nst, const T *u, T *v, T *s, hipStrea
   int64_t blocks = (W.n_det + wpb - 1) / wpb;
   QPX_REQUIRE(blocks < (1LL << 31), "too many determinants for one launch");
   const Ham &H = c->H;
+  if (c->ktiming) QPX_HIP(hipEventRecord(c->kev[0], st));
   switch (W.nint) {
     case 1: hipLaunchKernelGGL((sigma_generic_kernel<T, 1>), dim3((unsigned)blocks), dim3(64 * wpb), 0, st, A, Tab<T>::V(H), Tab<T>::hcore(H), u, v, s); break;
     case 2: hipLaunchKernelGGL((sigma_generic_kernel<T, 2>), dim3((unsigned)blocks), dim3(64 * wpb), 0, st, A, Tab<T>::V(H), Tab<T>::hcore(H), u, v, s); break;
@@ -558,6 +562,10 @@ static void launch_generic(qpx_ctx *c, int nst, const T *u, T *v, T *s, hipStrea
     default: hipLaunchKernelGGL((sigma_generic_kernel<T, 4>), dim3((unsigned)blocks), dim3(64 * wpb), 0, st, A, Tab<T>::V(H), Tab<T>::hcore(H), u, v, s); break;
   }
   QPX_HIP(hipGetLastError());
+  if (c->ktiming) {
+    QPX_HIP(hipEventRecord(c->kev[1], st));
+    QPX_HIP(hipEventRecord(c->kev[2], st));
+  }
 }
 
 namespace qpx {
@@ -677,5 +685,37 @@ extern "C" int qpx_debug_force_sigma_path(qpx_ctx *c, int path, int *last_path) 
     QPX_REQUIRE(path >= 0 && path <= 2, "path must be 0, 1 or 2");
     c->force_path = path;
     if (last_path) *last_path = c->sigma_path;
+  });
+}
+
+// benchmark hook: average duration (ms) of the sigma kernels over `reps`
+// calls, measured with events on the launch stream: ms[0] = alpha/beta +
+// same-spin alpha kernel (or the generic kernel), ms[1] = same-spin beta.
+extern "C" int qpx_debug_sigma_kernel_times(qpx_ctx *c, int prec, int nst, const void *u_t, void *v_t, void *s_t,
+                                            void *stream, float *ms, int reps) {
+  return qpx_guard(c, [&] {
+    QPX_HIP(hipSetDevice(c->device));
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    for (int i = 0; i < 3; ++i)
+      if (!c->kev[i]) QPX_HIP(hipEventCreate(&c->kev[i]));
+    double t0 = 0.0, t1 = 0.0;
+    c->ktiming = true;
+    try {
+      for (int r = 0; r < reps; ++r) {
+        sigma_device(c, prec, nst, u_t, v_t, s_t, st);
+        QPX_HIP(hipEventSynchronize(c->kev[2]));
+        float a = 0.f, b = 0.f;
+        QPX_HIP(hipEventElapsedTime(&a, c->kev[0], c->kev[1]));
+        QPX_HIP(hipEventElapsedTime(&b, c->kev[1], c->kev[2]));
+        t0 += a;
+        t1 += b;
+      }
+    } catch (...) {
+      c->ktiming = false;
+      throw;
+    }
+    c->ktiming = false;
+    ms[0] = (float)(t0 / reps);
+    ms[1] = (float)(t1 / reps);
   });
 }
