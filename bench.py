@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (the hot path is measured on one "
@@ -61,6 +63,14 @@ def main():
 
     import numpy as np
     import torch
+    if args.calib_copy:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")   # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes": src.numel() * 8}), flush=True)
+        return
     from quantum_package_amd.api import Context, FP32, FP64
     from quantum_package_amd.hamiltonian import fci_space, synthetic_hamiltonian
 
@@ -89,8 +99,11 @@ def main():
     u = torch.randn(n_det * nst, dtype=dt, device=dev, generator=g)
     v = torch.empty_like(u)
     s = torch.empty_like(u)
-    stream = torch.cuda.current_stream(dev)
+    # a dedicated stream (default priority): the kernels and the timing
+    # events must share it
+    stream = torch.cuda.Stream(device=dev)
     sptr = stream.cuda_stream
+    torch.cuda.synchronize()
 
     def step():
         ctx._check(ctx.L.qpx_sigma_device(ctx.ptr, prec, nst, u.data_ptr(), v.data_ptr(), s.data_ptr(), sptr))

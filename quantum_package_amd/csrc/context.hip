@@ -7,6 +7,7 @@
 // pair matrix V[P][R] (P, R = packed pair indices), so every kernel reads an
 // integral with one load and no index folding.
 #include <cstring>
+#include <cstdlib>
 #include <cmath>
 #include "qpx_internal.h"
 
@@ -44,6 +45,7 @@ extern "C" int qpx_create(int device, qpx_ctx **out) {
     hipDeviceProp_t prop;
     QPX_HIP(hipGetDeviceProperties(&prop, device));
     c->lds_per_block = (int)prop.sharedMemPerBlock;
+    if (const char *e = std::getenv("QPX_SIGMA_BS")) c->sigma_bs = std::atoi(e);
     c->n_cu = prop.multiProcessorCount;
   });
   if (rc) {

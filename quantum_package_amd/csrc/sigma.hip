@@ -66,9 +66,84 @@ struct DenseArgs {
   int n_a, n_b, M, npair;
 };
 
-// LDS layout: map[n_map * NST] | slice[M*M]
-template <typename T, int NI, int NST, int DPT>
-__global__ __launch_bounds__(1024) void sigma_dense_col_kernel(DenseArgs A, const T *__restrict__ V,
+// Same-spin partners of one string from its conn lists (singles and
+// doubles), 4-way unrolled so that the list loads of consecutive entries are
+// in flight together.  map is the LDS map of the current column/row, g the
+// other-spin Coulomb slice g(h,p) = sum_k (hp|kk).
+template <typename T, int NST>
+__device__ inline void same_spin_partners(const int *__restrict__ s_idx, const uint32_t *__restrict__ s_info,
+                                          const T *__restrict__ s_val, int s0, int s1,
+                                          const int *__restrict__ d_idx, const T *__restrict__ d_val, int d0,
+                                          int d1, const T *map, const T *g, int M, T *acc) {
+  int es = s0;
+  for (; es + 4 <= s1; es += 4) {
+    int ap[4];
+    uint32_t inf[4];
+    T sv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { ap[q] = s_idx[es + q]; inf[q] = s_info[es + q]; sv[q] = s_val[es + q]; }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      T gg = g[(inf[q] & 0xff) * M + ((inf[q] >> 8) & 0xff)];
+      T val = sv[q] + (((inf[q] >> 16) & 1) ? -gg : gg);
+#pragma unroll
+      for (int st = 0; st < NST; ++st) acc[st] += val * map[ap[q] * NST + st];
+    }
+  }
+  for (; es < s1; ++es) {
+    const int ap = s_idx[es];
+    const uint32_t inf = s_info[es];
+    T gg = g[(inf & 0xff) * M + ((inf >> 8) & 0xff)];
+    T val = s_val[es] + (((inf >> 16) & 1) ? -gg : gg);
+#pragma unroll
+    for (int st = 0; st < NST; ++st) acc[st] += val * map[ap * NST + st];
+  }
+  int ed = d0;
+  for (; ed + 4 <= d1; ed += 4) {
+    int ap[4];
+    T dv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { ap[q] = d_idx[ed + q]; dv[q] = d_val[ed + q]; }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int st = 0; st < NST; ++st) acc[st] += dv[q] * map[ap[q] * NST + st];
+  }
+  for (; ed < d1; ++ed) {
+    const int ap = d_idx[ed];
+    const T dv = d_val[ed];
+#pragma unroll
+    for (int st = 0; st < NST; ++st) acc[st] += dv * map[ap * NST + st];
+  }
+}
+
+// g(h,p) = sum_{k in string} (hp|kk) for all M*M (h,p), into LDS
+template <typename T, int NI>
+__device__ inline void coulomb_slice(const uint64_t *sw, const T *__restrict__ V, int npair, int M, T *out) {
+  for (int x = threadIdx.x; x < M * M; x += blockDim.x) {
+    int h = x / M, p = x - h * M;
+    const int64_t P = pair_index(h, p);
+    T g = T(0);
+#pragma unroll
+    for (int w = 0; w < NI; ++w) {
+      uint64_t y = sw[w];
+      while (y) {
+        int k = 64 * w + __ffsll((unsigned long long)y) - 1;
+        y &= y - 1;
+        g += V[P * npair + pair_index(k, k)];
+      }
+    }
+    out[x] = g;
+  }
+}
+
+// Column kernel: workgroup = (beta column b, chunk of 1024 of its
+// determinants), one determinant per thread.  The thread keeps the
+// alpha-singles of its alpha string in registers (packed a' | h<<16 |
+// p<<23 | phase<<30) for the whole beta-single loop.
+// LDS layout: map[n_a * NST] | slice[M*M]
+template <typename T, int NI, int NST, int MAXS, int BS>
+__global__ __launch_bounds__(BS) void sigma_dense_col_kernel(DenseArgs A, int nchunk, const T *__restrict__ V,
                                                                const T *__restrict__ as_val,
                                                                const T *__restrict__ ad_val,
                                                                const T *__restrict__ u, T *__restrict__ v,
@@ -76,21 +151,32 @@ __global__ __launch_bounds__(1024) void sigma_dense_col_kernel(DenseArgs A, cons
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *map = reinterpret_cast<T *>(lds_raw);
   T *slice = map + (size_t)A.n_a * NST;
-  const int b = blockIdx.x;
+  const int b = blockIdx.x / nchunk, chunk = blockIdx.x - b * nchunk;
   const int c0 = A.col_ptr[b], c1 = A.col_ptr[b + 1];
   const int n_col = c1 - c0;
   const int tid = threadIdx.x, nth = blockDim.x;
-  const int M = A.M, M2 = M * M;
-
-  int my_a[DPT];
-  T acc[DPT][NST], accs[DPT][NST];
-#pragma unroll
-  for (int d = 0; d < DPT; ++d) {
-    int e = tid + d * nth;
-    my_a[d] = (e < n_col) ? A.csc_a[c0 + e] : -1;
-#pragma unroll
-    for (int st = 0; st < NST; ++st) acc[d][st] = accs[d][st] = T(0);
+  const int M = A.M;
+  const int e = c0 + chunk * nth + tid;
+  const bool mine = e < c1;
+  const int a = mine ? A.csc_a[e] : 0;
+  int ns = 0, sbeg = 0;
+  if (mine) {
+    sbeg = A.as_ptr[a];
+    ns = A.as_ptr[a + 1] - sbeg;
   }
+  uint32_t L[MAXS];
+#pragma unroll
+  for (int j = 0; j < MAXS; ++j) {
+    uint32_t x = 0;
+    if (j < ns) {
+      uint32_t inf = A.as_info[sbeg + j];
+      x = (uint32_t)A.as_idx[sbeg + j] | ((inf & 0x7f) << 16) | (((inf >> 8) & 0x7f) << 23) | (((inf >> 16) & 1) << 30);
+    }
+    L[j] = x;
+  }
+  T acc[NST], accs[NST];
+#pragma unroll
+  for (int st = 0; st < NST; ++st) acc[st] = accs[st] = T(0);
   uint64_t bw[NI];
 #pragma unroll
   for (int w = 0; w < NI; ++w) bw[w] = A.bstr[(int64_t)b * NI + w];
@@ -106,36 +192,43 @@ __global__ __launch_bounds__(1024) void sigma_dense_col_kernel(DenseArgs A, cons
       for (int x = tid; x < A.n_a * NST; x += nth) map[x] = T(0);
       __syncthreads();
     }
-    for (int e = l0 + tid; e < l1; e += nth) {
-      int ai = A.csc_a[e];
+    if (l1 - l0 == A.n_a) {
+      // full column: contiguous copy
+      for (int x = tid; x < A.n_a * NST; x += nth) map[x] = u[(int64_t)l0 * NST + x];
+    } else {
+      for (int ee = l0 + tid; ee < l1; ee += nth) {
+        int ai = A.csc_a[ee];
 #pragma unroll
-      for (int st = 0; st < NST; ++st) map[ai * NST + st] = u[(int64_t)e * NST + st];
+        for (int st = 0; st < NST; ++st) map[ai * NST + st] = u[(int64_t)ee * NST + st];
+      }
     }
     const int64_t Pb = pair_index(hb, pb);
-    for (int x = tid; x < M2; x += nth) {
+    for (int x = tid; x < M * M; x += nth) {
       int h = x / M, p = x - h * M;
       slice[x] = V[pair_index(h, p) * A.npair + Pb];
     }
     __syncthreads();
+    // binfo has no bits above 16: xor-ing it in keeps the decode of L[j]
+    // inside this loop (hoisting it would multiply the live registers)
+    const uint32_t keep = binfo >> 24;
 #pragma unroll
-    for (int d = 0; d < DPT; ++d) {
-      const int a = my_a[d];
-      if (a < 0) continue;
-      for (int es = A.as_ptr[a]; es < A.as_ptr[a + 1]; ++es) {
-        const int ap = A.as_idx[es];
-        const uint32_t ai = A.as_info[es];
-        const int ha = ai & 0xff, pa = (ai >> 8) & 0xff;
+    for (int j = 0; j < MAXS; ++j) {
+      if (j < ns) {
+        const uint32_t x = L[j] ^ keep;
+        const int ap = x & 0xffff, ha = (x >> 16) & 0x7f, pa = (x >> 23) & 0x7f;
+        const bool neg = ((x >> 30) & 1) ^ phb;
         T val = slice[ha * M + pa];
-        const bool neg = ((ai >> 16) & 1) ^ phb;
         if (neg) val = -val;
         const bool flip = (ha == pb) && (pa == hb);
 #pragma unroll
         for (int st = 0; st < NST; ++st) {
           T uu = map[ap * NST + st];
-          acc[d][st] += val * uu;
-          if (flip) accs[d][st] += neg ? uu : -uu;
+          acc[st] += val * uu;
+          if (flip) accs[st] += neg ? uu : -uu;
         }
       }
+      // bound the number of LDS loads the scheduler keeps in flight
+      if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
     }
   }
 
@@ -145,72 +238,41 @@ __global__ __launch_bounds__(1024) void sigma_dense_col_kernel(DenseArgs A, cons
     for (int x = tid; x < A.n_a * NST; x += nth) map[x] = T(0);
     __syncthreads();
   }
-  for (int e = c0 + tid; e < c1; e += nth) {
-    int ai = A.csc_a[e];
+  for (int ee = c0 + tid; ee < c1; ee += nth) {
+    int ai = A.csc_a[ee];
 #pragma unroll
-    for (int st = 0; st < NST; ++st) map[ai * NST + st] = u[(int64_t)e * NST + st];
+    for (int st = 0; st < NST; ++st) map[ai * NST + st] = u[(int64_t)ee * NST + st];
   }
-  // g_b(h,p) = sum_{k in b} (hp|kk)
-  for (int x = tid; x < M2; x += nth) {
-    int h = x / M, p = x - h * M;
-    const int64_t P = pair_index(h, p);
-    T g = T(0);
-#pragma unroll
-    for (int w = 0; w < NI; ++w) {
-      uint64_t y = bw[w];
-      while (y) {
-        int k = 64 * w + __ffsll((unsigned long long)y) - 1;
-        y &= y - 1;
-        g += V[P * A.npair + pair_index(k, k)];
-      }
-    }
-    slice[x] = g;
-  }
+  coulomb_slice<T, NI>(bw, V, A.npair, M, slice);
   __syncthreads();
-#pragma unroll
-  for (int d = 0; d < DPT; ++d) {
-    const int a = my_a[d];
-    if (a < 0) continue;
-    for (int es = A.as_ptr[a]; es < A.as_ptr[a + 1]; ++es) {
-      const int ap = A.as_idx[es];
-      const uint32_t ai = A.as_info[es];
-      const int ha = ai & 0xff, pa = (ai >> 8) & 0xff;
-      T g = slice[ha * M + pa];
-      T val = as_val[es] + (((ai >> 16) & 1) ? -g : g);
-#pragma unroll
-      for (int st = 0; st < NST; ++st) acc[d][st] += val * map[ap * NST + st];
-    }
-    for (int ed = A.ad_ptr[a]; ed < A.ad_ptr[a + 1]; ++ed) {
-      const int ap = A.ad_idx[ed];
-      T val = ad_val[ed];
-#pragma unroll
-      for (int st = 0; st < NST; ++st) acc[d][st] += val * map[ap * NST + st];
-    }
-    const int e = c0 + tid + d * nth;
+  if (mine) {
+    same_spin_partners<T, NST>(A.as_idx, A.as_info, as_val, sbeg, sbeg + ns, A.ad_idx, ad_val, A.ad_ptr[a],
+                               A.ad_ptr[a + 1], map, slice, M, acc);
     const T hd = (T)A.hdiag[e], sd = (T)A.sdiag[e];
 #pragma unroll
     for (int st = 0; st < NST; ++st) {
       T uu = map[a * NST + st];
-      v[(int64_t)e * NST + st] = acc[d][st] + hd * uu;
-      s[(int64_t)e * NST + st] = accs[d][st] + sd * uu;
+      v[(int64_t)e * NST + st] = acc[st] + hd * uu;
+      s[(int64_t)e * NST + st] = accs[st] + sd * uu;
     }
   }
 }
 
-// same-spin beta part: one workgroup per alpha row a
-template <typename T, int NI, int NST, int DPT>
-__global__ __launch_bounds__(1024) void sigma_dense_row_kernel(DenseArgs A, const T *__restrict__ V,
+// same-spin beta part: workgroup = (alpha row a, chunk of 1024), one
+// determinant per thread; v += over the row's beta partners
+template <typename T, int NI, int NST>
+__global__ __launch_bounds__(1024) void sigma_dense_row_kernel(DenseArgs A, int nchunk, const T *__restrict__ V,
                                                                const T *__restrict__ bs_val,
                                                                const T *__restrict__ bd_val,
                                                                const T *__restrict__ u, T *__restrict__ v) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *map = reinterpret_cast<T *>(lds_raw);
   T *slice = map + (size_t)A.n_b * NST;
-  const int a = blockIdx.x;
+  const int a = blockIdx.x / nchunk, chunk = blockIdx.x - a * nchunk;
   const int r0 = A.row_ptr[a], r1 = A.row_ptr[a + 1];
   const int n_row = r1 - r0;
   const int tid = threadIdx.x, nth = blockDim.x;
-  const int M = A.M, M2 = M * M;
+  const int M = A.M;
   if (n_row != A.n_b) {
     for (int x = tid; x < A.n_b * NST; x += nth) map[x] = T(0);
     __syncthreads();
@@ -224,49 +286,19 @@ __global__ __launch_bounds__(1024) void sigma_dense_row_kernel(DenseArgs A, cons
   uint64_t aw[NI];
 #pragma unroll
   for (int w = 0; w < NI; ++w) aw[w] = A.astr[(int64_t)a * NI + w];
-  for (int x = tid; x < M2; x += nth) {
-    int h = x / M, p = x - h * M;
-    const int64_t P = pair_index(h, p);
-    T g = T(0);
-#pragma unroll
-    for (int w = 0; w < NI; ++w) {
-      uint64_t y = aw[w];
-      while (y) {
-        int k = 64 * w + __ffsll((unsigned long long)y) - 1;
-        y &= y - 1;
-        g += V[P * A.npair + pair_index(k, k)];
-      }
-    }
-    slice[x] = g;
-  }
+  coulomb_slice<T, NI>(aw, V, A.npair, M, slice);
   __syncthreads();
+  const int t = r0 + chunk * nth + tid;
+  if (t >= r1) return;
+  const int b = A.csr_b[t];
+  T acc[NST];
 #pragma unroll
-  for (int d = 0; d < DPT; ++d) {
-    const int t = r0 + tid + d * nth;
-    if (t >= r1) continue;
-    const int b = A.csr_b[t];
-    T acc[NST];
+  for (int st = 0; st < NST; ++st) acc[st] = T(0);
+  same_spin_partners<T, NST>(A.bs_idx, A.bs_info, bs_val, A.bs_ptr[b], A.bs_ptr[b + 1], A.bd_idx, bd_val,
+                             A.bd_ptr[b], A.bd_ptr[b + 1], map, slice, M, acc);
+  const int e = A.csr2csc[t];
 #pragma unroll
-    for (int st = 0; st < NST; ++st) acc[st] = T(0);
-    for (int es = A.bs_ptr[b]; es < A.bs_ptr[b + 1]; ++es) {
-      const int bp = A.bs_idx[es];
-      const uint32_t bi = A.bs_info[es];
-      const int hb = bi & 0xff, pb = (bi >> 8) & 0xff;
-      T g = slice[hb * M + pb];
-      T val = bs_val[es] + (((bi >> 16) & 1) ? -g : g);
-#pragma unroll
-      for (int st = 0; st < NST; ++st) acc[st] += val * map[bp * NST + st];
-    }
-    for (int ed = A.bd_ptr[b]; ed < A.bd_ptr[b + 1]; ++ed) {
-      const int bp = A.bd_idx[ed];
-      T val = bd_val[ed];
-#pragma unroll
-      for (int st = 0; st < NST; ++st) acc[st] += val * map[bp * NST + st];
-    }
-    const int e = A.csr2csc[t];
-#pragma unroll
-    for (int st = 0; st < NST; ++st) v[(int64_t)e * NST + st] += acc[st];
-  }
+  for (int st = 0; st < NST; ++st) v[(int64_t)e * NST + st] += acc[st];
 }
 
 // ---------------------------------------------------------------------------
@@ -496,45 +528,57 @@ static DenseArgs dense_args(qpx_ctx *c) {
   return A;
 }
 
-template <typename T, int NI, int NST, int DPT>
+template <typename T, int NI, int NST, int MAXS, int BS>
 static void launch_dense(qpx_ctx *c, const T *u, T *v, T *s, hipStream_t st) {
   DenseArgs A = dense_args(c);
   const Ham &H = c->H;
   const Wfn &W = c->W;
   size_t lds_col = ((size_t)W.n_a * NST + (size_t)H.M * H.M) * sizeof(T);
   size_t lds_row = ((size_t)W.n_b * NST + (size_t)H.M * H.M) * sizeof(T);
-  auto kc = sigma_dense_col_kernel<T, NI, NST, DPT>;
-  auto kr = sigma_dense_row_kernel<T, NI, NST, DPT>;
+  auto kc = sigma_dense_col_kernel<T, NI, NST, MAXS, BS>;
+  auto kr = sigma_dense_row_kernel<T, NI, NST>;
   QPX_HIP(hipFuncSetAttribute((const void *)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_col));
   QPX_HIP(hipFuncSetAttribute((const void *)kr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_row));
+  const int ncc = (W.max_col + BS - 1) / BS, ncr = (W.max_row + 1023) / 1024;
   if (c->ktiming) QPX_HIP(hipEventRecord(c->kev[0], st));
-  hipLaunchKernelGGL(kc, dim3(W.n_b), dim3(1024), lds_col, st, A, Tab<T>::V(H), Tab<T>::sval(W.ca),
+  hipLaunchKernelGGL(kc, dim3(W.n_b * ncc), dim3(BS), lds_col, st, A, ncc, Tab<T>::V(H), Tab<T>::sval(W.ca),
                      Tab<T>::dval(W.ca), u, v, s);
   QPX_HIP(hipGetLastError());
   if (c->ktiming) QPX_HIP(hipEventRecord(c->kev[1], st));
-  hipLaunchKernelGGL(kr, dim3(W.n_a), dim3(1024), lds_row, st, A, Tab<T>::V(H), Tab<T>::sval(W.cb),
+  hipLaunchKernelGGL(kr, dim3(W.n_a * ncr), dim3(1024), lds_row, st, A, ncr, Tab<T>::V(H), Tab<T>::sval(W.cb),
                      Tab<T>::dval(W.cb), u, v);
   QPX_HIP(hipGetLastError());
   if (c->ktiming) QPX_HIP(hipEventRecord(c->kev[2], st));
 }
 
+template <typename T, int NI, int NST, int BS>
+static void launch_dense_maxs_bs(qpx_ctx *c, int maxs, const T *u, T *v, T *s, hipStream_t st) {
+  if (maxs <= 16) launch_dense<T, NI, NST, 16, BS>(c, u, v, s, st);
+  else if (maxs <= 32) launch_dense<T, NI, NST, 32, BS>(c, u, v, s, st);
+  else if (maxs <= 48) launch_dense<T, NI, NST, 48, BS>(c, u, v, s, st);
+  else if (maxs <= 64) launch_dense<T, NI, NST, 64, BS>(c, u, v, s, st);
+  else launch_dense<T, NI, NST, 96, BS>(c, u, v, s, st);
+}
+
+// workgroup size of the column kernel: 1024 (16 waves, 128 VGPRs); 512 (8
+// waves, 256 VGPRs) avoids the few spills of long alpha-singles lists but
+// measured slower on the 14-orbital full-CI case (57.5 vs 45.4 ms);
+// QPX_SIGMA_BS overrides
 template <typename T, int NI, int NST>
-static void launch_dense_dpt(qpx_ctx *c, int dpt, const T *u, T *v, T *s, hipStream_t st) {
-  switch (dpt) {
-    case 1: launch_dense<T, NI, NST, 1>(c, u, v, s, st); break;
-    case 2: launch_dense<T, NI, NST, 2>(c, u, v, s, st); break;
-    case 3: launch_dense<T, NI, NST, 3>(c, u, v, s, st); break;
-    default: launch_dense<T, NI, NST, 4>(c, u, v, s, st); break;
-  }
+static void launch_dense_maxs(qpx_ctx *c, int maxs, const T *u, T *v, T *s, hipStream_t st) {
+  int bs = c->sigma_bs;
+  if (bs == 0) bs = 1024;
+  if (bs == 512) launch_dense_maxs_bs<T, NI, NST, 512>(c, maxs, u, v, s, st);
+  else launch_dense_maxs_bs<T, NI, NST, 1024>(c, maxs, u, v, s, st);
 }
 
 template <typename T, int NI>
-static void launch_dense_nst(qpx_ctx *c, int nst, int dpt, const T *u, T *v, T *s, hipStream_t st) {
+static void launch_dense_nst(qpx_ctx *c, int nst, int maxs, const T *u, T *v, T *s, hipStream_t st) {
   switch (nst) {
-    case 1: launch_dense_dpt<T, NI, 1>(c, dpt, u, v, s, st); break;
-    case 2: launch_dense_dpt<T, NI, 2>(c, dpt, u, v, s, st); break;
-    case 3: launch_dense_dpt<T, NI, 3>(c, dpt, u, v, s, st); break;
-    default: launch_dense_dpt<T, NI, 4>(c, dpt, u, v, s, st); break;
+    case 1: launch_dense_maxs<T, NI, 1>(c, maxs, u, v, s, st); break;
+    case 2: launch_dense_maxs<T, NI, 2>(c, maxs, u, v, s, st); break;
+    case 3: launch_dense_maxs<T, NI, 3>(c, maxs, u, v, s, st); break;
+    default: launch_dense_maxs<T, NI, 4>(c, maxs, u, v, s, st); break;
   }
 }
 
@@ -576,7 +620,7 @@ static bool dense_ok(qpx_ctx *c, int nst, size_t tsize) {
   const Wfn &W = c->W;
   const Ham &H = c->H;
   if (W.nint > 2 || nst > 4) return false;
-  if (W.max_col > 4096 || W.max_row > 4096) return false;
+  if (W.n_a > 65535 || H.M > 128 || W.ca.max_singles > 96) return false;
   size_t need_c = ((size_t)W.n_a * nst + (size_t)H.M * H.M) * tsize;
   size_t need_r = ((size_t)W.n_b * nst + (size_t)H.M * H.M) * tsize;
   size_t lim = (size_t)c->lds_per_block;
@@ -592,16 +636,15 @@ void sigma_device(qpx_ctx *c, int prec, int nst, const void *u, void *v, void *s
   if (c->force_path == 2) dense = false;
   if (c->force_path == 1) QPX_REQUIRE(dense, "dense-map sigma path not applicable to this wavefunction");
   const Wfn &W = c->W;
-  int dpt = std::max((W.max_col + 1023) / 1024, (W.max_row + 1023) / 1024);
-  if (dpt < 1) dpt = 1;
+  const int maxs = W.ca.max_singles;
   if (dense) {
     c->sigma_path = 1;
     if (prec == QPX_FP64) {
-      if (W.nint == 1) launch_dense_nst<double, 1>(c, nst, dpt, (const double *)u, (double *)v, (double *)s, st);
-      else launch_dense_nst<double, 2>(c, nst, dpt, (const double *)u, (double *)v, (double *)s, st);
+      if (W.nint == 1) launch_dense_nst<double, 1>(c, nst, maxs, (const double *)u, (double *)v, (double *)s, st);
+      else launch_dense_nst<double, 2>(c, nst, maxs, (const double *)u, (double *)v, (double *)s, st);
     } else {
-      if (W.nint == 1) launch_dense_nst<float, 1>(c, nst, dpt, (const float *)u, (float *)v, (float *)s, st);
-      else launch_dense_nst<float, 2>(c, nst, dpt, (const float *)u, (float *)v, (float *)s, st);
+      if (W.nint == 1) launch_dense_nst<float, 1>(c, nst, maxs, (const float *)u, (float *)v, (float *)s, st);
+      else launch_dense_nst<float, 2>(c, nst, maxs, (const float *)u, (float *)v, (float *)s, st);
     }
   } else {
     c->sigma_path = 2;
