@@ -40,6 +40,15 @@ EXTRA = [
     ("qpx_select_blocks", C.c_int, [c_vp, C.c_int, c_dp, C.c_int, C.c_int, c_dp, C.c_int64,
                                     C.POINTER(C.c_uint8), c_dp, c_u64p, c_dp, c_i64p, c_dp, c_i64p]),
     ("qpx_select_count_blocks", C.c_int, [c_vp, C.c_int, c_i64p]),
+    ("qpx_mlp_create", C.c_int, [C.c_int, C.c_int, C.c_uint64, C.POINTER(c_vp)]),
+    ("qpx_mlp_destroy", C.c_int, [c_vp]),
+    ("qpx_mlp_last_error", C.c_int, [c_vp, C.c_char_p, C.c_int]),
+    ("qpx_mlp_train_step", C.c_int, [c_vp, C.c_int, c_ip, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
+                                     C.POINTER(C.c_float)]),
+    ("qpx_mlp_predict", C.c_int, [c_vp, C.c_int, c_ip, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("qpx_mlp_param", C.c_int, [c_vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int64]),
+    ("qpx_mlp_gemm_check", C.c_int, [c_vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                     C.POINTER(C.c_float)]),
 ]
 
 _LIB = None

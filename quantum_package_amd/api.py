@@ -149,6 +149,37 @@ class Context:
         n = n_out.value
         return pt2, out_d[:n].copy(), out_v[:n].copy()
 
+    def select_blocks(self, coef_sorted, n_gen: int, n_sel: int, E0, N: int, keep=None):
+        """Selection restricted to the blocks with keep[b] != 0 (one byte per
+        block of the first n_gen generators, in the reference's block order;
+        None = all).  Returns (pt2, dets, vals, block_pt2[n_blocks, N_states],
+        block_info[n_blocks, 6] = gen, s1, h1, s2, h2 (1-based), status)."""
+        coef = np.asfortranarray(coef_sorted, dtype=np.float64)
+        if coef.ndim == 1:
+            coef = coef[:, None].copy(order="F")
+        ns = coef.shape[1]
+        nb = C.c_int64(0)
+        self._check(self.L.qpx_select_count_blocks(self.ptr, n_gen, C.byref(nb)))
+        nb = nb.value
+        E0 = np.ascontiguousarray(np.atleast_1d(E0), dtype=np.float64)
+        pt2 = np.zeros(ns)
+        out_d = np.zeros((max(N, 1), 2, self.n_int), dtype=np.uint64)
+        out_v = np.zeros(max(N, 1))
+        n_out = C.c_int64(0)
+        bpt2 = np.zeros((nb, ns))
+        binfo = np.zeros((nb, 6), dtype=np.int64)
+        kp = None
+        if keep is not None:
+            k = np.ascontiguousarray(keep, dtype=np.uint8)
+            if k.size != nb:
+                raise QpxError(f"keep mask has {k.size} entries, expected {nb}")
+            kp = k.ctypes.data_as(C.POINTER(C.c_uint8))
+        self._check(self.L.qpx_select_blocks(self.ptr, ns, _dp(coef), n_gen, n_sel, _dp(E0), N, kp, _dp(pt2),
+                                             out_d.ctypes.data_as(_lib.c_u64p), _dp(out_v), C.byref(n_out),
+                                             _dp(bpt2), binfo.ctypes.data_as(_lib.c_i64p)))
+        n = n_out.value
+        return pt2, out_d[:n].copy(), out_v[:n].copy(), bpt2, binfo
+
     def select_stats(self):
         st = np.zeros(4, dtype=np.int64)
         self._check(self.L.qpx_select_stats(self.ptr, st.ctypes.data_as(_lib.c_i64p)))
@@ -159,3 +190,85 @@ class Context:
         last = C.c_int(0)
         self._check(self.L.qpx_debug_force_sigma_path(self.ptr, path, C.byref(last)))
         return last.value
+
+
+# ---------------------------------------------------------------------------
+# block-proposal network (BF16 MFMA kernels in libqpx)
+# ---------------------------------------------------------------------------
+_c_fp = C.POINTER(C.c_float)
+PARAM_NAMES = ["W1", "W2", "W3", "b1", "b2", "b3", "w4", "emb", "b4"]
+
+
+class BlockMLP:
+    """MLP over (generator, hole-pair) block descriptors predicting the
+    block's log|EN-PT2 contribution|.  desc (n, 4) int32: h1, h2 (0-based),
+    spin class (0 aa, 1 bb, 2 ab), generator excitation degree from HF;
+    fdesc (n, 3) float32: h_11, h_22, log|c_gen|."""
+
+    def __init__(self, n_orb: int, seed: int = 1, device: int = 0):
+        self.L = _lib.load()
+        self.ptr = C.c_void_p()
+        rc = self.L.qpx_mlp_create(device, n_orb, seed, C.byref(self.ptr))
+        if rc != 0:
+            raise QpxError(f"qpx_mlp_create failed ({rc})")
+        self.n_orb = n_orb
+        self.shapes = {"W1": (80, 256), "W2": (256, 256), "W3": (256, 256), "b1": (256,), "b2": (256,),
+                       "b3": (256,), "w4": (256,), "emb": (n_orb, 32), "b4": (1,)}
+
+    def _check(self, rc):
+        if rc != 0:
+            buf = C.create_string_buffer(1024)
+            self.L.qpx_mlp_last_error(self.ptr, buf, 1024)
+            raise QpxError(buf.value.decode(errors="replace"))
+
+    def close(self):
+        if self.ptr:
+            self.L.qpx_mlp_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _prep(desc, fdesc):
+        d = np.ascontiguousarray(desc, dtype=np.int32)
+        f = np.ascontiguousarray(fdesc, dtype=np.float32)
+        return d, f
+
+    def train_step(self, desc, fdesc, target, lr):
+        d, f = self._prep(desc, fdesc)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        loss = C.c_float(0)
+        self._check(self.L.qpx_mlp_train_step(self.ptr, d.shape[0], d.ctypes.data_as(_lib.c_ip),
+                                              f.ctypes.data_as(_c_fp), t.ctypes.data_as(_c_fp), lr,
+                                              C.byref(loss)))
+        return loss.value
+
+    def predict(self, desc, fdesc):
+        d, f = self._prep(desc, fdesc)
+        out = np.zeros(d.shape[0], dtype=np.float32)
+        self._check(self.L.qpx_mlp_predict(self.ptr, d.shape[0], d.ctypes.data_as(_lib.c_ip),
+                                           f.ctypes.data_as(_c_fp), out.ctypes.data_as(_c_fp)))
+        return out
+
+    def get(self, name, grad=False):
+        shp = self.shapes[name]
+        a = np.zeros(int(np.prod(shp)), dtype=np.float32)
+        self._check(self.L.qpx_mlp_param(self.ptr, PARAM_NAMES.index(name), int(grad), 0,
+                                         a.ctypes.data_as(_c_fp), a.size))
+        return a.reshape(shp)
+
+    def set(self, name, value):
+        a = np.ascontiguousarray(value, dtype=np.float32).ravel()
+        self._check(self.L.qpx_mlp_param(self.ptr, PARAM_NAMES.index(name), 0, 1, a.ctypes.data_as(_c_fp), a.size))
+
+    def gemm_check(self, A, B):
+        A = np.ascontiguousarray(A, dtype=np.float32)
+        B = np.ascontiguousarray(B, dtype=np.float32)
+        Cm = np.zeros((A.shape[0], B.shape[1]), dtype=np.float32)
+        self._check(self.L.qpx_mlp_gemm_check(self.ptr, A.shape[0], B.shape[1], A.shape[1], A.ctypes.data_as(_c_fp),
+                                              B.ctypes.data_as(_c_fp), Cm.ctypes.data_as(_c_fp)))
+        return Cm

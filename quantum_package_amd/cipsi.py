@@ -1,0 +1,210 @@
+"""CIPSI driver: the loop of the reference's ``fci_zmq`` program
+(plugins/Full_CI_ZMQ/fci_zmq.irp.f:57-170) around the GPU hot path.
+
+Per iteration:
+  1. diagonalize_CI (davidson_diag_hs2 on the current psi);
+  2. psi_det_sorted: determinants by decreasing state-averaged weight
+     (src/Determinants/determinants.irp.f:228-293); generators / selectors
+     are the leading determinants up to threshold_generators /
+     threshold_selectors of the norm (plugins/Generators_full/generators.irp.f:3-23,
+     plugins/Selectors_full/selectors.irp.f:3-24);
+  3. ZMQ_selection(to_select, pt2) with E0 = psi_energy
+     (plugins/Full_CI_ZMQ/energy.irp.f:9-24), to_select = N_det/2+1 when
+     s2_eig else N_det (fci_zmq.irp.f:148-156);
+  4. fill_H_apply_buffer_no_selection + copy_H_apply_buffer_to_wf: the new
+     determinants enter psi with zero coefficients and psi is normalised
+     (src/Determinants/H_apply.irp.f:113-198,275-316);
+  5. make_s2_eigenfunction when s2_eig or N_states > 1: every determinant
+     sharing an occupation pattern with psi is added
+     (src/Determinants/occ_pattern.irp.f:251-309).
+
+The backend is libqpx (quantum_package_amd.api.Context); the same loop runs
+on any object exposing set_wavefunction / davidson_diag_hs2 / ZMQ_selection.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from itertools import combinations
+
+import numpy as np
+
+
+def popcount64(x: np.ndarray) -> np.ndarray:
+    x = x.astype(np.uint64)
+    c = np.zeros(x.shape, dtype=np.int64)
+    for sh in range(0, 64, 8):
+        c += _POP8[((x >> np.uint64(sh)) & np.uint64(0xFF)).astype(np.int64)]
+    return c
+
+
+_POP8 = np.array([bin(i).count("1") for i in range(256)], dtype=np.int64)
+
+
+def det_keys(dets: np.ndarray) -> np.ndarray:
+    """one hashable row per determinant (structured void view)"""
+    d = np.ascontiguousarray(dets.reshape(dets.shape[0], int(np.prod(dets.shape[1:]))))
+    return d.view(np.dtype((np.void, d.dtype.itemsize * d.shape[1]))).ravel()
+
+
+def sort_by_weight(dets, coef, state_weights=None):
+    """psi_det_sorted: decreasing state-averaged weight, stable."""
+    ns = coef.shape[1]
+    w = np.ones(ns) / ns if state_weights is None else np.asarray(state_weights)
+    contrib = (coef ** 2) @ w
+    contrib = contrib / contrib.sum()
+    order = np.argsort(-contrib, kind="stable")
+    return dets[order], coef[order], contrib[order]
+
+
+def n_leading(contrib_sorted, threshold, strict=False):
+    """number of leading determinants whose cumulated weight reaches threshold"""
+    c = np.cumsum(contrib_sorted)
+    idx = np.nonzero(c > threshold if strict else c >= threshold)[0]
+    return int(idx[0]) + 1 if idx.size else len(contrib_sorted)
+
+
+def make_s2_eigenfunction(dets: np.ndarray, n_alpha: int) -> np.ndarray:
+    """Determinants sharing an occupation pattern with dets and not in dets
+    (occ_pattern.irp.f:251-309).  dets (N, 2, n_int) uint64."""
+    n_int = dets.shape[2]
+    a = dets[:, 0, :]
+    b = dets[:, 1, :]
+    docc = a & b
+    socc = a ^ b
+    pat = np.concatenate([docc, socc], axis=1)
+    upat = np.unique(pat, axis=0)
+    nd = popcount64(upat[:, :n_int]).sum(axis=1)
+    ns = popcount64(upat[:, n_int:]).sum(axis=1)
+    out = []
+    for n_s in np.unique(ns):
+        sel = ns == n_s
+        P = upat[sel]
+        na_s = n_alpha - nd[sel]
+        for k in np.unique(na_s):
+            Q = P[na_s == k]
+            if n_s == 0 or k < 0 or k > n_s:
+                continue
+            # orbital bit positions of the singly occupied orbitals, ascending
+            pos = np.zeros((Q.shape[0], n_s), dtype=np.int64)
+            cur = np.zeros(Q.shape[0], dtype=np.int64)
+            one = np.uint64(1)
+            for w in range(n_int):
+                x = Q[:, n_int + w].copy()
+                while True:
+                    nz = x != 0
+                    if not nz.any():
+                        break
+                    with np.errstate(over="ignore"):
+                        lb = x & (~x + one)
+                        p = popcount64(lb - one) + 64 * w
+                        rows = np.nonzero(nz)[0]
+                        pos[rows, cur[rows]] = p[rows]
+                        cur[rows] += 1
+                        x = np.where(nz, x & (x - one), x)
+            for comb in combinations(range(n_s), int(k)):
+                alpha = Q[:, :n_int].copy()
+                beta = Q[:, :n_int].copy()
+                for j in range(n_s):
+                    w = pos[:, j] // 64
+                    bitv = np.left_shift(np.uint64(1), (pos[:, j] % 64).astype(np.uint64))
+                    tgt = alpha if j in comb else beta
+                    for ww in range(n_int):
+                        m = w == ww
+                        tgt[m, ww] |= bitv[m]
+                out.append(np.stack([alpha, beta], axis=1))
+    if not out:
+        return np.zeros((0, 2, n_int), dtype=np.uint64)
+    cand = np.concatenate(out, axis=0)
+    ck = det_keys(cand)
+    inpsi = np.isin(ck, det_keys(dets))
+    cand = cand[~inpsi]
+    if cand.shape[0] == 0:
+        return cand
+    _, first = np.unique(det_keys(cand), return_index=True)
+    return cand[np.sort(first)]
+
+
+@dataclass
+class CipsiResult:
+    dets: np.ndarray
+    coef: np.ndarray
+    energies: np.ndarray
+    pt2: np.ndarray
+    history: list = field(default_factory=list)
+
+
+def cipsi(ctx, H, n_det_max: int, n_states: int = 1, n_states_diag: int = 4, s2_eig: bool = True,
+          threshold_davidson: float = 1e-10, threshold_generators: float = 0.99,
+          threshold_selectors: float = 0.999, pt2_max: float = 1e-4, guess_dets=None, log=print,
+          mixed: bool = False):
+    """Run the CIPSI loop on the GPU context ctx from HF (or guess_dets)."""
+    from .hamiltonian import hf_determinant
+    M, na, nb = H.M, H.n_alpha, H.n_beta
+    if guess_dets is None:
+        dets = hf_determinant(M, na, nb)[None, :, :]
+    else:
+        dets = np.asarray(guess_dets, dtype=np.uint64)
+    ctx.set_hamiltonian(H)
+    coef = np.zeros((dets.shape[0], n_states))
+    coef[0, 0] = 1.0
+    history = []
+    pt2 = np.full(n_states, -np.inf)
+    energies = None
+
+    def diagonalize(dets, coef):
+        ctx.set_wavefunction(dets)
+        n = dets.shape[0]
+        if n < 3 * n_states_diag:
+            # small psi: dense diagonalisation (N_det <= N_det_max_jacobi)
+            Hm = np.zeros((n, n))
+            for k in range(n):
+                e = np.zeros(n)
+                e[k] = 1.0
+                Hm[:, k], _ = ctx.H_S2_u_0_nstates(e)
+            w, V = np.linalg.eigh(0.5 * (Hm + Hm.T))
+            return w[:n_states], V[:, :n_states]
+        g = np.zeros((n, n_states))
+        g[:, :] = coef[:, :n_states]
+        for k in range(n_states):
+            if not np.any(g[:, k]):
+                g[k % n, k] = 1.0
+        e, s2, u, _ = ctx.davidson_diag_hs2(g, N_st=n_states, N_st_diag=max(n_states_diag, n_states),
+                                            threshold=threshold_davidson, s2_eig=s2_eig, mixed=mixed)
+        return e[:n_states], u[:, :n_states]
+
+    while True:
+        t0 = time.time()
+        energies, coef = diagonalize(dets, coef)
+        n_det = dets.shape[0]
+        rec = dict(n_det=n_det, energies=energies.tolist(), pt2=None, t_diag=time.time() - t0)
+        if n_det >= n_det_max or np.max(np.abs(pt2)) <= pt2_max:
+            history.append(rec)
+            log(f"cipsi: N_det={n_det} E={energies} (stop)")
+            break
+        dets_s, coef_s, contrib = sort_by_weight(dets, coef)
+        n_gen = n_leading(contrib, threshold_generators)
+        n_sel = n_det if threshold_generators >= 1.0 else max(n_leading(contrib, threshold_selectors, True), n_gen)
+        ctx.set_wavefunction(dets_s)
+        E0 = np.array([ctx.u_0_H_u_0(coef_s[:, k])[0] for k in range(n_states)])
+        to_select = (n_det // 2 + 1) if s2_eig else n_det
+        to_select = min(to_select, n_det_max - n_det)
+        t1 = time.time()
+        pt2, new_dets, vals = ctx.ZMQ_selection(np.asfortranarray(coef_s), n_gen, n_sel, E0, to_select)
+        rec.update(pt2=pt2.tolist(), n_gen=n_gen, n_sel=n_sel, n_new=int(new_dets.shape[0]),
+                   t_sel=time.time() - t1)
+        # new determinants with zero coefficients, psi normalised
+        dets = np.concatenate([dets_s, new_dets], axis=0)
+        coef = np.concatenate([coef_s, np.zeros((new_dets.shape[0], n_states))], axis=0)
+        if s2_eig or n_states > 1:
+            extra = make_s2_eigenfunction(dets, na)
+            if extra.shape[0]:
+                dets = np.concatenate([dets, extra], axis=0)
+                coef = np.concatenate([coef, np.zeros((extra.shape[0], n_states))], axis=0)
+            rec["n_s2_added"] = int(extra.shape[0])
+        coef /= np.linalg.norm(coef, axis=0)
+        history.append(rec)
+        log(f"cipsi: N_det={n_det} E={energies} PT2={pt2} gen={n_gen} sel={n_sel} -> {dets.shape[0]} "
+            f"({rec['t_diag']:.1f}s diag, {rec['t_sel']:.1f}s sel)")
+    return CipsiResult(dets=dets, coef=coef, energies=energies, pt2=pt2, history=history)

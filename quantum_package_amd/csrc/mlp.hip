@@ -1,0 +1,548 @@
+// mlp.hip -- the block-proposal network: a small MLP trained and run with
+// hand-written BF16 MFMA kernels (FP32 accumulation) on gfx950.
+//
+// No counterpart in the reference: it ranks the (generator, hole-pair)
+// blocks of select_singles_and_doubles (plugins/Full_CI_ZMQ/selection.irp.f:
+// 412-575) -- the same unit the reference's `subset` / fragment_count filter
+// acts on (:561-562) -- by predicted log|PT2 contribution|.
+//
+// Network: x (D_IN features) -> 256 -> 256 -> 256 (ReLU) -> 1.  Features of a
+// block: learned 32-d embeddings of h1 and h2 (one table over orbitals),
+// h_11 and h_22 orbital energies, log|c| of the generator, one-hot spin
+// class (aa, bb, ab), one-hot excitation degree of the generator from HF
+// (0..4), zero padding to D_IN = 80.
+//
+// GEMM: C(MxN) = A(MxK) B(KxN), A and B bf16 with arbitrary row/column
+// strides (so A^T, B^T need no copies), C fp32; workgroup tile 64x64, four
+// wavefronts each owning 32x32 = 2x2 tiles of v_mfma_f32_16x16x16_bf16.
+// Lane layout of the 16x16x16 MFMA: lane l holds A[l%16][4(l/16)+0..3],
+// B[4(l/16)+0..3][l%16] and D[4(l/16)+0..3][l%16].
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include <hip/hip_bf16.h>
+#include "qpx_internal.h"
+
+using namespace qpx;
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ inline uint16_t f2bf(float x) {  // round to nearest even
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7fffffff) > 0x7f800000) return 0x7fc0;  // NaN
+  u += 0x7fff + ((u >> 16) & 1);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float bf2f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
+
+struct GemmArgs {
+  const uint16_t *A;  // bf16
+  int64_t a_rs, a_cs; // A(i,k) = A[i*a_rs + k*a_cs]
+  const uint16_t *B;
+  int64_t b_rs, b_cs; // B(k,j) = B[k*b_rs + j*b_cs]
+  float *C;           // C(i,j) = C[i*c_rs + j]
+  int64_t c_rs;
+  int M, N, K;
+  const float *bias;  // per column j (nullable)
+  int relu;           // epilogue: C = relu(C + bias)
+  uint16_t *Cb;       // optional bf16 copy of C (same layout)
+  float beta;         // C = acc + beta * C_old (gradient accumulation)
+};
+
+constexpr int TM = 64, TN = 64, TK = 16;
+
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
+  __shared__ uint16_t As[TK][TM + 4];  // k-major
+  __shared__ uint16_t Bs[TK][TN + 4];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int i0 = blockIdx.y * TM, j0 = blockIdx.x * TN;
+  const int wi = (wv >> 1) * 32, wj = (wv & 1) * 32;
+  floatx4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < g.K; k0 += TK) {
+    __syncthreads();
+    // stage A tile (TM x TK) and B tile (TK x TN): 1024 elements each
+    for (int x = tid; x < TM * TK; x += 256) {
+      int ii = x / TK, kk = x - ii * TK;
+      int gi = i0 + ii, gk = k0 + kk;
+      As[kk][ii] = (gi < g.M && gk < g.K) ? g.A[gi * g.a_rs + gk * g.a_cs] : (uint16_t)0;
+    }
+    for (int x = tid; x < TK * TN; x += 256) {
+      int kk = x / TN, jj = x - kk * TN;
+      int gk = k0 + kk, gj = j0 + jj;
+      Bs[kk][jj] = (gk < g.K && gj < g.N) ? g.B[gk * g.b_rs + gj * g.b_cs] : (uint16_t)0;
+    }
+    __syncthreads();
+    const int r = lane & 15, q = (lane >> 4) * 4;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      bf16x4 av;
+      av[0] = (short)As[q + 0][wi + 16 * a + r];
+      av[1] = (short)As[q + 1][wi + 16 * a + r];
+      av[2] = (short)As[q + 2][wi + 16 * a + r];
+      av[3] = (short)As[q + 3][wi + 16 * a + r];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        bf16x4 bv;
+        bv[0] = (short)Bs[q + 0][wj + 16 * b + r];
+        bv[1] = (short)Bs[q + 1][wj + 16 * b + r];
+        bv[2] = (short)Bs[q + 2][wj + 16 * b + r];
+        bv[3] = (short)Bs[q + 3][wj + 16 * b + r];
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[a][b], 0, 0, 0);
+      }
+    }
+  }
+  // epilogue: lane holds D[q+0..3][r] of each 16x16 tile
+  const int r = lane & 15, q = (lane >> 4) * 4;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        int gi = i0 + wi + 16 * a + q + t, gj = j0 + wj + 16 * b + r;
+        if (gi < g.M && gj < g.N) {
+          float v = acc[a][b][t];
+          float *c = g.C + gi * g.c_rs + gj;
+          if (g.beta != 0.f) v += g.beta * (*c);
+          if (g.bias) v += g.bias[gj];
+          if (g.relu) v = v > 0.f ? v : 0.f;
+          *c = v;
+          if (g.Cb) g.Cb[gi * g.c_rs + gj] = f2bf(v);
+        }
+      }
+}
+
+__global__ void to_bf16_kernel(const float *__restrict__ x, uint16_t *__restrict__ y, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = f2bf(x[i]);
+}
+
+// features of the blocks: desc = (h1, h2 (0-based), spin class 0..2,
+// degree 0..4) int32 x4; fdesc = (eps1, eps2, log|c|) float x3
+__global__ void features_kernel(const int *__restrict__ desc, const float *__restrict__ fdesc,
+                                const float *__restrict__ emb, int n, int din, float *__restrict__ X,
+                                uint16_t *__restrict__ Xb) {
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < (int64_t)n * din;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    int i = (int)(x / din), f = (int)(x - (int64_t)i * din);
+    const int *d = desc + 4 * i;
+    float v = 0.f;
+    if (f < 32) v = emb[d[0] * 32 + f];
+    else if (f < 64) v = emb[d[1] * 32 + (f - 32)];
+    else if (f < 67) v = fdesc[3 * i + (f - 64)];
+    else if (f < 70) v = (d[2] == f - 67) ? 1.f : 0.f;
+    else if (f < 75) v = (d[3] == f - 70) ? 1.f : 0.f;
+    X[x] = v;
+    Xb[x] = f2bf(v);
+  }
+}
+
+// y = H3 w4 + b4 (one output), loss partials and dy = 2 (y - t) / n
+__global__ void head_kernel(const float *__restrict__ H, const float *__restrict__ w4, float b4, int n, int hdim,
+                            const float *__restrict__ t, float *__restrict__ y, float *__restrict__ dy,
+                            float *__restrict__ loss_part) {
+  __shared__ float red[256];
+  float l = 0.f;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float s = b4;
+    for (int k = 0; k < hdim; ++k) s += H[(int64_t)i * hdim + k] * w4[k];
+    y[i] = s;
+    if (t) {
+      float e = s - t[i];
+      l += e * e;
+      if (dy) dy[i] = 2.f * e / n;
+    }
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && loss_part) loss_part[blockIdx.x] = red[0];
+}
+
+// dZ = (dy w4^T) masked by H > 0; bf16 copy; dw4 = H^T dy partial sums
+__global__ void head_back_kernel(const float *__restrict__ H, const float *__restrict__ w4,
+                                 const float *__restrict__ dy, int n, int hdim, float *__restrict__ dZ,
+                                 uint16_t *__restrict__ dZb) {
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < (int64_t)n * hdim;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    int i = (int)(x / hdim), k = (int)(x - (int64_t)i * hdim);
+    float v = H[x] > 0.f ? dy[i] * w4[k] : 0.f;
+    dZ[x] = v;
+    dZb[x] = f2bf(v);
+  }
+}
+
+__global__ void relu_mask_kernel(float *__restrict__ dH, const float *__restrict__ H, int64_t n,
+                                 uint16_t *__restrict__ dHb) {
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+    float v = H[x] > 0.f ? dH[x] : 0.f;
+    dH[x] = v;
+    dHb[x] = f2bf(v);
+  }
+}
+
+// column sums of a (n x m) row-major fp32 matrix (bias gradients), and the
+// head weight gradient dw4[k] = sum_i H[i,k] dy[i]
+__global__ void colsum_kernel(const float *__restrict__ A, const float *__restrict__ w, int n, int m,
+                              float *__restrict__ out) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  float s = 0.f;
+  for (int i = 0; i < n; ++i) s += A[(int64_t)i * m + k] * (w ? w[i] : 1.f);
+  out[k] = s;
+}
+
+__global__ void emb_grad_kernel(const int *__restrict__ desc, const float *__restrict__ dX, int n, int din,
+                                float *__restrict__ demb) {
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < (int64_t)n * 64;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    int i = (int)(x / 64), f = (int)(x - (int64_t)i * 64);
+    int h = f < 32 ? desc[4 * i] : desc[4 * i + 1];
+    atomicAdd(&demb[h * 32 + (f & 31)], dX[(int64_t)i * din + f]);
+  }
+}
+
+__global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                            float *__restrict__ v, int64_t n, float lr, float b1, float b2, float eps, float c1,
+                            float c2, uint16_t *__restrict__ pb) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float gi = g[i];
+    float mi = b1 * m[i] + (1.f - b1) * gi;
+    float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    float upd = lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    float pi = p[i] - upd;
+    p[i] = pi;
+    if (pb) pb[i] = f2bf(pi);
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct qpx_mlp {
+  int din = 80, hid = 256, n_orb = 0;
+  hipStream_t st = nullptr;
+  int device = 0;
+  // parameters (fp32 master) and bf16 copies; layer l: W (in x out) row-major
+  DBuf<float> W1, W2, W3, b1, b2, b3, w4, emb;
+  DBuf<uint16_t> W1b, W2b, W3b;
+  DBuf<float> gW1, gW2, gW3, gb1, gb2, gb3, gw4, gemb, gb4;
+  DBuf<float> mW1, mW2, mW3, mb1, mb2, mb3, mw4, memb, vW1, vW2, vW3, vb1, vb2, vb3, vw4, vemb;
+  float b4 = 0.f, mb4 = 0.f, vb4 = 0.f;
+  int step = 0;
+  // activations for a batch
+  int cap = 0;
+  DBuf<float> X, Z1, Z2, Z3, y, dy, dZ3, dZ2, dZ1, dX, lossp;
+  DBuf<uint16_t> Xb, H1b, H2b, H3b, dZ3b, dZ2b, dZ1b;
+  DBuf<int> desc;
+  DBuf<float> fdesc, tgt;
+  std::string err;
+};
+
+namespace {
+
+void gemm(hipStream_t st, const uint16_t *A, int64_t ars, int64_t acs, const uint16_t *B, int64_t brs, int64_t bcs,
+          float *C, int64_t crs, int M, int N, int K, const float *bias, int relu, uint16_t *Cb, float beta) {
+  GemmArgs g{A, ars, acs, B, brs, bcs, C, crs, M, N, K, bias, relu, Cb, beta};
+  dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
+  hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, st, g);
+  QPX_HIP(hipGetLastError());
+}
+
+void ensure(qpx_mlp *m, int n) {
+  if (n <= m->cap) return;
+  int c = n;
+  m->X.alloc((size_t)c * m->din); m->Xb.alloc((size_t)c * m->din); m->dX.alloc((size_t)c * m->din);
+  m->Z1.alloc((size_t)c * m->hid); m->Z2.alloc((size_t)c * m->hid); m->Z3.alloc((size_t)c * m->hid);
+  m->H1b.alloc((size_t)c * m->hid); m->H2b.alloc((size_t)c * m->hid); m->H3b.alloc((size_t)c * m->hid);
+  m->dZ1.alloc((size_t)c * m->hid); m->dZ2.alloc((size_t)c * m->hid); m->dZ3.alloc((size_t)c * m->hid);
+  m->dZ1b.alloc((size_t)c * m->hid); m->dZ2b.alloc((size_t)c * m->hid); m->dZ3b.alloc((size_t)c * m->hid);
+  m->y.alloc(c); m->dy.alloc(c); m->lossp.alloc(1024);
+  m->desc.alloc((size_t)4 * c); m->fdesc.alloc((size_t)3 * c); m->tgt.alloc(c);
+  m->cap = c;
+}
+
+int grid1(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 16384); }
+
+void forward(qpx_mlp *m, int n) {
+  hipStream_t st = m->st;
+  const int D = m->din, Hd = m->hid;
+  hipLaunchKernelGGL(features_kernel, dim3(grid1((int64_t)n * D)), dim3(256), 0, st, m->desc.p, m->fdesc.p, m->emb.p,
+                     n, D, m->X.p, m->Xb.p);
+  // Z = act(X W + b): A = X (n x D, row-major), B = W (D x H, row-major)
+  gemm(st, m->Xb.p, D, 1, m->W1b.p, Hd, 1, m->Z1.p, Hd, n, Hd, D, m->b1.p, 1, m->H1b.p, 0.f);
+  gemm(st, m->H1b.p, Hd, 1, m->W2b.p, Hd, 1, m->Z2.p, Hd, n, Hd, Hd, m->b2.p, 1, m->H2b.p, 0.f);
+  gemm(st, m->H2b.p, Hd, 1, m->W3b.p, Hd, 1, m->Z3.p, Hd, n, Hd, Hd, m->b3.p, 1, m->H3b.p, 0.f);
+}
+
+float train_step(qpx_mlp *m, int n, float lr) {
+  hipStream_t st = m->st;
+  const int D = m->din, Hd = m->hid;
+  forward(m, n);
+  int nb = std::min(1024, (n + 255) / 256);
+  hipLaunchKernelGGL(head_kernel, dim3(nb), dim3(256), 0, st, m->Z3.p, m->w4.p, m->b4, n, Hd, m->tgt.p, m->y.p,
+                     m->dy.p, m->lossp.p);
+  // head grads
+  hipLaunchKernelGGL(colsum_kernel, dim3((Hd + 255) / 256), dim3(256), 0, st, m->Z3.p, m->dy.p, n, Hd, m->gw4.p);
+  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(256), 0, st, m->dy.p, (const float *)nullptr, n, 1, m->gb4.p);
+  hipLaunchKernelGGL(head_back_kernel, dim3(grid1((int64_t)n * Hd)), dim3(256), 0, st, m->Z3.p, m->w4.p, m->dy.p,
+                     n, Hd, m->dZ3.p, m->dZ3b.p);
+  // layer 3: dW3 = H2^T dZ3 (H x H, K = n); db3 = colsum dZ3; dH2 = dZ3 W3^T
+  gemm(st, m->H2b.p, 1, Hd, m->dZ3b.p, Hd, 1, m->gW3.p, Hd, Hd, Hd, n, nullptr, 0, nullptr, 0.f);
+  hipLaunchKernelGGL(colsum_kernel, dim3((Hd + 255) / 256), dim3(256), 0, st, m->dZ3.p, (const float *)nullptr, n,
+                     Hd, m->gb3.p);
+  gemm(st, m->dZ3b.p, Hd, 1, m->W3b.p, 1, Hd, m->dZ2.p, Hd, n, Hd, Hd, nullptr, 0, nullptr, 0.f);
+  hipLaunchKernelGGL(relu_mask_kernel, dim3(grid1((int64_t)n * Hd)), dim3(256), 0, st, m->dZ2.p, m->Z2.p,
+                     (int64_t)n * Hd, m->dZ2b.p);
+  // layer 2
+  gemm(st, m->H1b.p, 1, Hd, m->dZ2b.p, Hd, 1, m->gW2.p, Hd, Hd, Hd, n, nullptr, 0, nullptr, 0.f);
+  hipLaunchKernelGGL(colsum_kernel, dim3((Hd + 255) / 256), dim3(256), 0, st, m->dZ2.p, (const float *)nullptr, n,
+                     Hd, m->gb2.p);
+  gemm(st, m->dZ2b.p, Hd, 1, m->W2b.p, 1, Hd, m->dZ1.p, Hd, n, Hd, Hd, nullptr, 0, nullptr, 0.f);
+  hipLaunchKernelGGL(relu_mask_kernel, dim3(grid1((int64_t)n * Hd)), dim3(256), 0, st, m->dZ1.p, m->Z1.p,
+                     (int64_t)n * Hd, m->dZ1b.p);
+  // layer 1: dW1 = X^T dZ1 (D x H); dX = dZ1 W1^T (n x D)
+  gemm(st, m->Xb.p, 1, D, m->dZ1b.p, Hd, 1, m->gW1.p, Hd, D, Hd, n, nullptr, 0, nullptr, 0.f);
+  hipLaunchKernelGGL(colsum_kernel, dim3((Hd + 255) / 256), dim3(256), 0, st, m->dZ1.p, (const float *)nullptr, n,
+                     Hd, m->gb1.p);
+  gemm(st, m->dZ1b.p, Hd, 1, m->W1b.p, 1, Hd, m->dX.p, D, n, D, Hd, nullptr, 0, nullptr, 0.f);
+  QPX_HIP(hipMemsetAsync(m->gemb.p, 0, sizeof(float) * m->emb.n, st));
+  hipLaunchKernelGGL(emb_grad_kernel, dim3(grid1((int64_t)n * 64)), dim3(256), 0, st, m->desc.p, m->dX.p, n, D,
+                     m->gemb.p);
+  QPX_HIP(hipGetLastError());
+  if (lr > 0.f) {
+    m->step++;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
+    auto ad = [&](DBuf<float> &p, DBuf<float> &g, DBuf<float> &mm, DBuf<float> &vv, uint16_t *pb) {
+      hipLaunchKernelGGL(adam_kernel, dim3(grid1((int64_t)p.n)), dim3(256), 0, st, p.p, g.p, mm.p, vv.p,
+                         (int64_t)p.n, lr, b1, b2, eps, c1, c2, pb);
+    };
+    ad(m->W1, m->gW1, m->mW1, m->vW1, m->W1b.p);
+    ad(m->W2, m->gW2, m->mW2, m->vW2, m->W2b.p);
+    ad(m->W3, m->gW3, m->mW3, m->vW3, m->W3b.p);
+    ad(m->b1, m->gb1, m->mb1, m->vb1, nullptr);
+    ad(m->b2, m->gb2, m->mb2, m->vb2, nullptr);
+    ad(m->b3, m->gb3, m->mb3, m->vb3, nullptr);
+    ad(m->w4, m->gw4, m->mw4, m->vw4, nullptr);
+    ad(m->emb, m->gemb, m->memb, m->vemb, nullptr);
+    QPX_HIP(hipGetLastError());
+    float gb4 = 0.f;
+    QPX_HIP(hipMemcpyAsync(&gb4, m->gb4.p, sizeof(float), hipMemcpyDeviceToHost, st));
+    QPX_HIP(hipStreamSynchronize(st));
+    m->mb4 = b1 * m->mb4 + (1 - b1) * gb4;
+    m->vb4 = b2 * m->vb4 + (1 - b2) * gb4 * gb4;
+    m->b4 -= lr * (m->mb4 / c1) / (std::sqrt(m->vb4 / c2) + eps);
+  }
+  std::vector<float> lp(nb);
+  QPX_HIP(hipMemcpyAsync(lp.data(), m->lossp.p, sizeof(float) * nb, hipMemcpyDeviceToHost, st));
+  QPX_HIP(hipStreamSynchronize(st));
+  double l = 0.0;
+  for (float x : lp) l += x;
+  return (float)(l / n);
+}
+
+template <typename F>
+int mguard(qpx_mlp *m, F &&f) {
+  if (!m) return 1;
+  try {
+    f();
+    return 0;
+  } catch (const qpx::Error &e) {
+    m->err = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    m->err = e.what();
+    return 3;
+  }
+}
+
+uint64_t sm64(uint64_t &s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI (no counterpart in the reference)
+// ---------------------------------------------------------------------------
+extern "C" int qpx_mlp_create(int device, int n_orb, uint64_t seed, qpx_mlp **out) {
+  if (!out) return 1;
+  qpx_mlp *m = new qpx_mlp();
+  int rc = mguard(m, [&] {
+    QPX_REQUIRE(n_orb > 0 && n_orb <= 256, "n_orb must be in 1..256");
+    QPX_HIP(hipSetDevice(device));
+    m->device = device;
+    m->n_orb = n_orb;
+    QPX_HIP(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
+    const int D = m->din, Hd = m->hid;
+    uint64_t s = seed;
+    auto init = [&](DBuf<float> &p, size_t n, float scale) {
+      std::vector<float> h(n);
+      for (auto &x : h) {
+        double u1 = 1.0 - (double)(sm64(s) >> 11) / 9007199254740992.0, u2 = (double)(sm64(s) >> 11) / 9007199254740992.0;
+        x = (float)(scale * std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2));
+      }
+      p.upload(h, m->st);
+    };
+    init(m->W1, (size_t)D * Hd, std::sqrt(2.0f / 75.f));
+    init(m->W2, (size_t)Hd * Hd, std::sqrt(2.0f / Hd));
+    init(m->W3, (size_t)Hd * Hd, std::sqrt(2.0f / Hd));
+    init(m->w4, Hd, std::sqrt(1.0f / Hd));
+    init(m->emb, (size_t)n_orb * 32, 0.3f);
+    std::vector<float> z(Hd, 0.f);
+    m->b1.upload(z, m->st);
+    m->b2.upload(z, m->st);
+    m->b3.upload(z, m->st);
+    m->W1b.alloc((size_t)D * Hd); m->W2b.alloc((size_t)Hd * Hd); m->W3b.alloc((size_t)Hd * Hd);
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(64), dim3(256), 0, m->st, m->W1.p, m->W1b.p, (int64_t)D * Hd);
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(64), dim3(256), 0, m->st, m->W2.p, m->W2b.p, (int64_t)Hd * Hd);
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(64), dim3(256), 0, m->st, m->W3.p, m->W3b.p, (int64_t)Hd * Hd);
+    auto zero = [&](DBuf<float> &p, size_t n) {
+      p.alloc(n);
+      QPX_HIP(hipMemsetAsync(p.p, 0, sizeof(float) * n, m->st));
+    };
+    zero(m->gW1, (size_t)D * Hd); zero(m->gW2, (size_t)Hd * Hd); zero(m->gW3, (size_t)Hd * Hd);
+    zero(m->gb1, Hd); zero(m->gb2, Hd); zero(m->gb3, Hd); zero(m->gw4, Hd); zero(m->gb4, 1);
+    zero(m->gemb, (size_t)n_orb * 32);
+    zero(m->mW1, (size_t)D * Hd); zero(m->mW2, (size_t)Hd * Hd); zero(m->mW3, (size_t)Hd * Hd);
+    zero(m->vW1, (size_t)D * Hd); zero(m->vW2, (size_t)Hd * Hd); zero(m->vW3, (size_t)Hd * Hd);
+    zero(m->mb1, Hd); zero(m->mb2, Hd); zero(m->mb3, Hd); zero(m->mw4, Hd);
+    zero(m->vb1, Hd); zero(m->vb2, Hd); zero(m->vb3, Hd); zero(m->vw4, Hd);
+    zero(m->memb, (size_t)n_orb * 32); zero(m->vemb, (size_t)n_orb * 32);
+    QPX_HIP(hipGetLastError());
+    QPX_HIP(hipStreamSynchronize(m->st));
+  });
+  if (rc) {
+    delete m;
+    return rc;
+  }
+  *out = m;
+  return 0;
+}
+
+extern "C" int qpx_mlp_destroy(qpx_mlp *m) {
+  if (!m) return 0;
+  (void)hipSetDevice(m->device);
+  if (m->st) {
+    (void)hipStreamSynchronize(m->st);
+    (void)hipStreamDestroy(m->st);
+  }
+  delete m;
+  return 0;
+}
+
+extern "C" int qpx_mlp_last_error(qpx_mlp *m, char *buf, int len) {
+  if (!m || !buf || len <= 0) return 1;
+  std::strncpy(buf, m->err.c_str(), (size_t)len - 1);
+  buf[len - 1] = 0;
+  return 0;
+}
+
+// One Adam step on a batch.  desc (n x 4 int32: h1, h2 0-based, spin class,
+// degree), fdesc (n x 3 float: eps1, eps2, log|c|), target (n) = label.
+// lr = 0 computes loss and gradients only.  *loss = mean squared error.
+extern "C" int qpx_mlp_train_step(qpx_mlp *m, int n, const int *desc, const float *fdesc, const float *target,
+                                  float lr, float *loss) {
+  return mguard(m, [&] {
+    QPX_HIP(hipSetDevice(m->device));
+    QPX_REQUIRE(n > 0, "empty batch");
+    ensure(m, n);
+    QPX_HIP(hipMemcpyAsync(m->desc.p, desc, sizeof(int) * 4 * n, hipMemcpyHostToDevice, m->st));
+    QPX_HIP(hipMemcpyAsync(m->fdesc.p, fdesc, sizeof(float) * 3 * n, hipMemcpyHostToDevice, m->st));
+    QPX_HIP(hipMemcpyAsync(m->tgt.p, target, sizeof(float) * n, hipMemcpyHostToDevice, m->st));
+    float l = train_step(m, n, lr);
+    if (loss) *loss = l;
+  });
+}
+
+// Predictions for n blocks (processed in chunks of up to 65536).
+extern "C" int qpx_mlp_predict(qpx_mlp *m, int n, const int *desc, const float *fdesc, float *out) {
+  return mguard(m, [&] {
+    QPX_HIP(hipSetDevice(m->device));
+    const int chunk = 65536;
+    for (int i0 = 0; i0 < n; i0 += chunk) {
+      int c = std::min(chunk, n - i0);
+      ensure(m, c);
+      QPX_HIP(hipMemcpyAsync(m->desc.p, desc + 4 * (size_t)i0, sizeof(int) * 4 * c, hipMemcpyHostToDevice, m->st));
+      QPX_HIP(hipMemcpyAsync(m->fdesc.p, fdesc + 3 * (size_t)i0, sizeof(float) * 3 * c, hipMemcpyHostToDevice, m->st));
+      forward(m, c);
+      int nb = std::min(1024, (c + 255) / 256);
+      hipLaunchKernelGGL(head_kernel, dim3(nb), dim3(256), 0, m->st, m->Z3.p, m->w4.p, m->b4, c, m->hid,
+                         (const float *)nullptr, m->y.p, (float *)nullptr, (float *)nullptr);
+      QPX_HIP(hipGetLastError());
+      QPX_HIP(hipMemcpyAsync(out + i0, m->y.p, sizeof(float) * c, hipMemcpyDeviceToHost, m->st));
+    }
+    QPX_HIP(hipStreamSynchronize(m->st));
+  });
+}
+
+// Parameter exchange (fp32), for checkpoints and the gradient checks:
+// which = 0 W1 (80x256), 1 W2, 2 W3 (256x256), 3 b1, 4 b2, 5 b3, 6 w4 (256),
+// 7 emb (n_orb x 32), 8 b4 (1); the matching gradients with grad = 1.
+extern "C" int qpx_mlp_param(qpx_mlp *m, int which, int grad, int set, float *data, int64_t n) {
+  return mguard(m, [&] {
+    QPX_HIP(hipSetDevice(m->device));
+    DBuf<float> *P[8] = {&m->W1, &m->W2, &m->W3, &m->b1, &m->b2, &m->b3, &m->w4, &m->emb};
+    DBuf<float> *G[8] = {&m->gW1, &m->gW2, &m->gW3, &m->gb1, &m->gb2, &m->gb3, &m->gw4, &m->gemb};
+    if (which == 8) {
+      QPX_REQUIRE(n == 1, "b4 has one element");
+      if (grad) {
+        QPX_HIP(hipMemcpyAsync(data, m->gb4.p, sizeof(float), hipMemcpyDeviceToHost, m->st));
+        QPX_HIP(hipStreamSynchronize(m->st));
+      } else if (set) {
+        m->b4 = data[0];
+      } else {
+        data[0] = m->b4;
+      }
+      return;
+    }
+    QPX_REQUIRE(which >= 0 && which < 8, "unknown parameter");
+    DBuf<float> &b = grad ? *G[which] : *P[which];
+    QPX_REQUIRE((size_t)n == b.n, "parameter size mismatch");
+    if (set) {
+      QPX_REQUIRE(!grad, "gradients are read-only");
+      QPX_HIP(hipMemcpyAsync(b.p, data, sizeof(float) * n, hipMemcpyHostToDevice, m->st));
+      uint16_t *bb = which == 0 ? m->W1b.p : which == 1 ? m->W2b.p : which == 2 ? m->W3b.p : nullptr;
+      if (bb) hipLaunchKernelGGL(to_bf16_kernel, dim3(64), dim3(256), 0, m->st, b.p, bb, (int64_t)n);
+    } else {
+      QPX_HIP(hipMemcpyAsync(data, b.p, sizeof(float) * n, hipMemcpyDeviceToHost, m->st));
+    }
+    QPX_HIP(hipStreamSynchronize(m->st));
+  });
+}
+
+// Plain bf16 GEMM C = A B (row-major A n x k, B k x m, fp32 inputs rounded
+// to bf16), for the lane-layout and accuracy checks of the MFMA tiles.
+extern "C" int qpx_mlp_gemm_check(qpx_mlp *m, int M, int N, int K, const float *A, const float *B, float *C) {
+  return mguard(m, [&] {
+    QPX_HIP(hipSetDevice(m->device));
+    DBuf<float> a, b, c;
+    DBuf<uint16_t> ab, bb;
+    a.upload(A, (size_t)M * K, m->st);
+    b.upload(B, (size_t)K * N, m->st);
+    ab.alloc((size_t)M * K);
+    bb.alloc((size_t)K * N);
+    c.alloc((size_t)M * N);
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(64), dim3(256), 0, m->st, a.p, ab.p, (int64_t)M * K);
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(64), dim3(256), 0, m->st, b.p, bb.p, (int64_t)K * N);
+    gemm(m->st, ab.p, K, 1, bb.p, N, 1, c.p, N, M, N, K, nullptr, 0, nullptr, 0.f);
+    QPX_HIP(hipMemcpyAsync(C, c.p, sizeof(float) * M * N, hipMemcpyDeviceToHost, m->st));
+    QPX_HIP(hipStreamSynchronize(m->st));
+  });
+}

@@ -151,7 +151,13 @@ __global__ __launch_bounds__(BS) void sigma_dense_col_kernel(DenseArgs A, int nc
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *map = reinterpret_cast<T *>(lds_raw);
   T *slice = map + (size_t)A.n_a * NST;
-  const int b = blockIdx.x / nchunk, chunk = blockIdx.x - b * nchunk;
+  // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so
+  // logical block = (id mod 8) * (grid/8) + id/8 keeps the chunks of one
+  // column, and neighbouring columns, on one XCD and its L2
+  const int per_xcd = gridDim.x >> 3;
+  const int logical = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  if (logical >= A.n_b * nchunk) return;
+  const int b = logical / nchunk, chunk = logical - b * nchunk;
   const int c0 = A.col_ptr[b], c1 = A.col_ptr[b + 1];
   const int n_col = c1 - c0;
   const int tid = threadIdx.x, nth = blockDim.x;
@@ -268,7 +274,10 @@ __global__ __launch_bounds__(1024) void sigma_dense_row_kernel(DenseArgs A, int 
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *map = reinterpret_cast<T *>(lds_raw);
   T *slice = map + (size_t)A.n_b * NST;
-  const int a = blockIdx.x / nchunk, chunk = blockIdx.x - a * nchunk;
+  const int per_xcd = gridDim.x >> 3;
+  const int logical = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  if (logical >= A.n_a * nchunk) return;
+  const int a = logical / nchunk, chunk = logical - a * nchunk;
   const int r0 = A.row_ptr[a], r1 = A.row_ptr[a + 1];
   const int n_row = r1 - r0;
   const int tid = threadIdx.x, nth = blockDim.x;
@@ -541,11 +550,12 @@ static void launch_dense(qpx_ctx *c, const T *u, T *v, T *s, hipStream_t st) {
   QPX_HIP(hipFuncSetAttribute((const void *)kr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_row));
   const int ncc = (W.max_col + BS - 1) / BS, ncr = (W.max_row + 1023) / 1024;
   if (c->ktiming) QPX_HIP(hipEventRecord(c->kev[0], st));
-  hipLaunchKernelGGL(kc, dim3(W.n_b * ncc), dim3(BS), lds_col, st, A, ncc, Tab<T>::V(H), Tab<T>::sval(W.ca),
+  const int gc = ((W.n_b * ncc + 7) / 8) * 8, gr = ((W.n_a * ncr + 7) / 8) * 8;  // multiples of the 8 XCDs
+  hipLaunchKernelGGL(kc, dim3(gc), dim3(BS), lds_col, st, A, ncc, Tab<T>::V(H), Tab<T>::sval(W.ca),
                      Tab<T>::dval(W.ca), u, v, s);
   QPX_HIP(hipGetLastError());
   if (c->ktiming) QPX_HIP(hipEventRecord(c->kev[1], st));
-  hipLaunchKernelGGL(kr, dim3(W.n_a * ncr), dim3(1024), lds_row, st, A, ncr, Tab<T>::V(H), Tab<T>::sval(W.cb),
+  hipLaunchKernelGGL(kr, dim3(gr), dim3(1024), lds_row, st, A, ncr, Tab<T>::V(H), Tab<T>::sval(W.cb),
                      Tab<T>::dval(W.cb), u, v);
   QPX_HIP(hipGetLastError());
   if (c->ktiming) QPX_HIP(hipEventRecord(c->kev[2], st));

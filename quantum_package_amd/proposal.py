@@ -1,0 +1,149 @@
+"""Learned block proposal for CIPSI selection.
+
+The exact selection (select_singles_and_doubles,
+plugins/Full_CI_ZMQ/selection.irp.f:258-578) spends its time in
+(generator, hole-pair) blocks; the reference already lets a caller run a
+subset of them (`subset` / fragment_count, :561-562).  Here a small MLP
+(BlockMLP, BF16 MFMA kernels) learns, from the per-block EN-PT2 sums that
+the exact selection itself produces, which blocks carry the PT2; only the
+blocks it proposes are enumerated, exactly.  A periodic full sweep measures
+the PT2 the proposal missed, and the full enumeration stays the fallback.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+LOG_FLOOR = float(np.log(1e-14))
+
+
+def block_order(na: int, nb: int):
+    """(s1, i1, s2, i2) of every block of one generator, in the reference's
+    loop order (selection.irp.f:412-420): s1 = 1,2; i1 descending; s2 >= s1;
+    i2 descending down to i1+1 (same spin) or 1.  i1/i2 index the
+    generator's sorted hole list of that spin (1-based)."""
+    out = []
+    nh = {1: na, 2: nb}
+    for s1 in (1, 2):
+        for i1 in range(nh[s1], 0, -1):
+            for s2 in range(s1, 3):
+                ib = i1 + 1 if s1 == s2 else 1
+                for i2 in range(nh[s2], ib - 1, -1):
+                    out.append((s1, i1, s2, i2))
+    return np.array(out, dtype=np.int64)
+
+
+def occupied_lists(dets: np.ndarray, n_occ: int, spin: int) -> np.ndarray:
+    """sorted 1-based occupied orbitals (n_det, n_occ) of one spin"""
+    n, _, n_int = dets.shape
+    out = np.zeros((n, n_occ), dtype=np.int64)
+    cur = np.zeros(n, dtype=np.int64)
+    one = np.uint64(1)
+    from .cipsi import popcount64
+    for w in range(n_int):
+        x = dets[:, spin, w].copy()
+        while True:
+            nz = x != 0
+            if not nz.any():
+                break
+            with np.errstate(over="ignore"):
+                lb = x & (~x + one)
+                p = popcount64(lb - one) + 64 * w + 1
+                rows = np.nonzero(nz)[0]
+                out[rows, cur[rows]] = p[rows]
+                cur[rows] += 1
+                x = np.where(nz, x & (x - one), x)
+    return out
+
+
+def block_descriptors(H, dets_sorted: np.ndarray, coef_sorted: np.ndarray, n_gen: int):
+    """desc (n_blocks, 4) int32 (h1, h2 0-based, spin class, degree) and
+    fdesc (n_blocks, 3) float32 (h_11, h_22, log|c|) for the blocks of the
+    first n_gen generators, in qpx_select_blocks order."""
+    from .hamiltonian import hf_determinant
+    from .cipsi import popcount64
+    na, nb = H.n_alpha, H.n_beta
+    gens = dets_sorted[:n_gen]
+    occ = {1: occupied_lists(gens, na, 0), 2: occupied_lists(gens, nb, 1)}
+    pat = block_order(na, nb)
+    nbk = pat.shape[0]
+    s1, i1, s2, i2 = pat[:, 0], pat[:, 1], pat[:, 2], pat[:, 3]
+    h1 = np.empty((n_gen, nbk), dtype=np.int64)
+    h2 = np.empty((n_gen, nbk), dtype=np.int64)
+    for s in (1, 2):
+        m1 = s1 == s
+        h1[:, m1] = occ[s][:, i1[m1] - 1]
+        m2 = s2 == s
+        h2[:, m2] = occ[s][:, i2[m2] - 1]
+    spin = np.where(s1 != s2, 2, s1 - 1)
+    hf = hf_determinant(H.M, na, nb)
+    deg = (popcount64(gens[:, 0, :] & ~hf[0][None, :]).sum(axis=1) +
+           popcount64(gens[:, 1, :] & ~hf[1][None, :]).sum(axis=1))
+    deg = np.minimum(deg, 4)
+    logc = np.log(np.abs(coef_sorted[:n_gen, 0]) + 1e-300)
+    desc = np.empty((n_gen * nbk, 4), dtype=np.int32)
+    desc[:, 0] = (h1 - 1).ravel()
+    desc[:, 1] = (h2 - 1).ravel()
+    desc[:, 2] = np.tile(spin, n_gen)
+    desc[:, 3] = np.repeat(deg, nbk)
+    hd = np.diag(H.h)
+    fdesc = np.empty((n_gen * nbk, 3), dtype=np.float32)
+    fdesc[:, 0] = hd[desc[:, 0]]
+    fdesc[:, 1] = hd[desc[:, 1]]
+    fdesc[:, 2] = np.repeat(logc, nbk)
+    return desc, fdesc
+
+
+def labels_from_blocks(block_pt2: np.ndarray) -> np.ndarray:
+    """log|e| of each block (state 0), floored"""
+    return np.maximum(np.log(np.abs(block_pt2[:, 0]) + 1e-300), LOG_FLOOR).astype(np.float32)
+
+
+class Trainer:
+    """Standardised regression of block labels with Adam on the GPU."""
+
+    def __init__(self, mlp, lr=1e-3, batch=4096, seed=0):
+        self.mlp = mlp
+        self.lr = lr
+        self.batch = batch
+        self.rng = np.random.default_rng(seed)
+        self.mu = None
+        self.sd = None
+        self.fmu = None
+        self.fsd = None
+
+    def _norm_f(self, fdesc):
+        return ((fdesc - self.fmu) / self.fsd).astype(np.float32)
+
+    def fit(self, desc, fdesc, y, epochs=3, log=None):
+        if self.mu is None:
+            self.mu, self.sd = float(y.mean()), float(y.std() + 1e-6)
+            self.fmu = fdesc.mean(axis=0)
+            self.fsd = fdesc.std(axis=0) + 1e-6
+        t = ((y - self.mu) / self.sd).astype(np.float32)
+        f = self._norm_f(fdesc)
+        n = desc.shape[0]
+        losses = []
+        for ep in range(epochs):
+            perm = self.rng.permutation(n)
+            for b0 in range(0, n - self.batch + 1 if n >= self.batch else 1, self.batch):
+                idx = perm[b0:b0 + self.batch]
+                losses.append(self.mlp.train_step(desc[idx], f[idx], t[idx], self.lr))
+            if log:
+                log(f"mlp epoch {ep}: loss {np.mean(losses[-50:]):.4f}")
+        return losses
+
+    def predict(self, desc, fdesc):
+        return self.mlp.predict(desc, self._norm_f(fdesc)) * self.sd + self.mu
+
+
+def choose_blocks(pred_log: np.ndarray, drop_fraction: float = 2e-3):
+    """keep the blocks with the largest predicted |e| until the predicted
+    contribution of the dropped blocks is <= drop_fraction of the total"""
+    w = np.exp(pred_log.astype(np.float64))
+    order = np.argsort(w)  # ascending
+    cum = np.cumsum(w[order])
+    tot = cum[-1]
+    n_drop = int(np.searchsorted(cum, drop_fraction * tot, side="right"))
+    keep = np.ones(pred_log.size, dtype=np.uint8)
+    keep[order[:n_drop]] = 0
+    return keep
