@@ -13,8 +13,18 @@ The JSON line also carries:
                 H_S2_u_0_nstates_openmp_work, the reference itself cannot be
                 built here) timed on a bounded, strided sample of the same
                 rows on this host's cores;
-  pt2           EN-PT2 candidates/s of the selection kernel on a bounded
-                generator sample (when --pt2 is given).
+  davidson      davidson_diag_hs2 on the same space (3 roots, 4 vectors,
+                threshold 1e-10): plain FP64 against mixed precision (FP32
+                correction-vector sigma, FP64 everything else), wall time,
+                sigma calls and |dE| per root;
+  pt2           CIPSI selection + EN-PT2 candidates/s (FP64) on the
+                128-orbital, 12 alpha + 12 beta electron system (N_int = 2):
+                a selected wavefunction of --pt2-ndet determinants (one
+                selection step from HF, then diagonalised), the leading
+                generators by weight timed on the GPU and, on a smaller
+                generator sample, on the CPU oracle.
+Legs after the first are skipped with --no-davidson / --no-pt2; a leg that
+fails records its error in its own field and the line is still printed.
 """
 from __future__ import annotations
 
@@ -53,6 +63,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--no-davidson", action="store_true")
+    ap.add_argument("--no-pt2", action="store_true")
+    ap.add_argument("--pt2-ndet", type=int, default=100000)
+    ap.add_argument("--pt2-seconds", type=float, default=20.0)
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -182,8 +196,130 @@ def main():
                                          f"FP64 4-vector sigma call, reference algorithm restated in C/OpenMP "
                                          f"schedule(dynamic,64); {dt1:.1f} s"}
         log(f"bench: cpu oracle {rows / dt1:.1f} dets/s on {nthr} threads")
-    print(json.dumps(out), flush=True)
+    del u, v, s
+    torch.cuda.empty_cache()
+    if not args.no_davidson:
+        out["davidson"] = _guarded(davidson_leg, ctx, H, dets, log)
     ctx.close()
+    if not args.no_pt2:
+        out["pt2"] = _guarded(pt2_leg, args, log)
+    print(json.dumps(out), flush=True)
+
+
+def _guarded(fn, *a):
+    try:
+        return fn(*a)
+    except Exception as e:   # the leg's failure is part of the record
+        return {"error": f"{type(e).__name__}: {e}"}
+
+
+def davidson_leg(ctx, H, dets, log):
+    """FP64 vs mixed-precision davidson_diag_hs2, 3 roots on the full-CI space."""
+    import numpy as np
+    from quantum_package_amd.hamiltonian import hf_determinant
+    n = dets.shape[0]
+    hf = hf_determinant(H.M, H.n_alpha, H.n_beta)
+    g = np.zeros((n, 3))
+    i_hf = np.where(np.all(dets.reshape(n, -1) == hf.ravel()[None, :], axis=1))[0][0]
+    g[i_hf, 0] = 1.0
+    g[:, 1:] = np.random.default_rng(1002).standard_normal((n, 2))
+    res = {}
+    for name, mixed in (("fp64", False), ("mixed", True)):
+        t0 = time.time()
+        e, s2, u, ns = ctx.davidson_diag_hs2(g, N_st=3, N_st_diag=4, threshold=1e-10, s2_eig=True, mixed=mixed,
+                                             switch_threshold=1e-6)
+        dt = time.time() - t0
+        res[name] = {"seconds": dt, "energies": e[:3].tolist(), "s2": s2[:3].tolist(),
+                     "sigma_calls_fp64": ns[0], "sigma_calls_fp32": ns[1]}
+        log(f"bench: davidson {name}: {dt:.2f} s, E = {e[:3]}, sigma calls {ns}")
+    d = np.abs(np.array(res["mixed"]["energies"]) - np.array(res["fp64"]["energies"]))
+    res["abs_dE_per_root"] = d.tolist()
+    res["speedup_mixed"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
+    res["workload"] = ("davidson_diag_hs2, 3 roots + 1 extra vector, threshold 1e-10, s2_eig, HF + seeded random "
+                       "guess, same 9,018,009-determinant space")
+    return res
+
+
+def _selected_psi(ctx, H, n_target, log):
+    """HF, one selection step of n_target-1 determinants, diagonalised:
+    returns (dets, coef) in psi_det_sorted order and E."""
+    import numpy as np
+    from quantum_package_amd.cipsi import sort_by_weight
+    from quantum_package_amd.hamiltonian import hf_determinant
+    hf = hf_determinant(H.M, H.n_alpha, H.n_beta)[None]
+    ctx.set_wavefunction(hf)
+    E0 = ctx.u_0_H_u_0(np.ones(1))
+    _, new, _ = ctx.ZMQ_selection(np.ones((1, 1), order="F"), 1, 1, E0, n_target - 1)
+    dets = np.concatenate([hf, new])
+    ctx.set_wavefunction(dets)
+    g = np.zeros((dets.shape[0], 1))
+    g[0, 0] = 1.0
+    e, _, u, ns = ctx.davidson_diag_hs2(g, N_st=1, N_st_diag=4, threshold=1e-8, s2_eig=False)
+    ds, cs, _ = sort_by_weight(dets, u[:, :1])
+    log(f"bench: selected psi {dets.shape[0]} dets, E = {e[0]:.10f} ({ns} sigma calls)")
+    return ds, np.asfortranarray(cs), e[0]
+
+
+def pt2_leg(args, log):
+    """Selection + EN-PT2 throughput, candidates/s."""
+    import numpy as np
+    from quantum_package_amd.api import Context
+    from quantum_package_amd.hamiltonian import synthetic_hamiltonian
+    M, na, nb, seed = 128, 12, 12, 1004
+    t0 = time.time()
+    H = synthetic_hamiltonian(M, na, nb, seed=seed)
+    ctx = Context(0)
+    ctx.set_hamiltonian(H)
+    log(f"bench: pt2 system M={M} {na}a{nb}b set up in {time.time() - t0:.1f}s")
+    ds, cs, _ = _selected_psi(ctx, H, args.pt2_ndet, log)
+    n = ds.shape[0]
+    ctx.set_wavefunction(ds)
+    E0 = ctx.u_0_H_u_0(cs[:, 0])
+
+    def run(n_gen):
+        t = time.time()
+        pt2, sel, _ = ctx.ZMQ_selection(cs, n_gen, n, E0, n)
+        return time.time() - t, pt2, ctx.select_stats(), sel.shape[0]
+
+    dt, _, st, _ = run(min(n, 64))          # warm-up and calibration
+    rate_gen = min(n, 64) / max(dt, 1e-3)
+    n_gen = int(min(n, max(64, rate_gen * args.pt2_seconds)))
+    dt, pt2, st, n_new = run(n_gen)
+    res = {"value": st["candidates"] / dt, "unit": "candidates/s", "dtype": "f64", "seconds": dt,
+           "generators": n_gen, "selectors": n, "candidates": st["candidates"], "blocks": st["blocks"],
+           "selected": n_new, "pt2": pt2.tolist(),
+           "workload": f"selection + EN-PT2, synthetic Hamiltonian seed {seed}: {M} MOs, {na} alpha + {nb} beta "
+                       f"electrons, N_int=2; psi = {n} determinants (HF + one selection step, diagonalised); the "
+                       f"{n_gen} leading generators by weight against all {n} selectors, to_select = N_det"}
+    log(f"bench: pt2 {n_gen} generators: {dt:.2f} s, {st['candidates']} candidates, "
+        f"{res['value']:.3e} candidates/s")
+    if not args.no_cpu:
+        # the reference's task farm: (generator, fragment) tasks, fragment
+        # count n_alpha^2 (selection.irp.f:561-562), leading generators first
+        from oracle import oracle as O
+        ham = O.Ham(H)
+        nfrag = na * na
+
+        def cpu(k):
+            tasks = np.array([[1 + i // nfrag, 1 + i % nfrag] for i in range(k)], dtype=np.int32)
+            t = time.time()
+            r = O.selection(ham, ds, cs, int(tasks[-1, 0]), n, E0, n, nthreads=args.cpu_threads, tasks=tasks)
+            return time.time() - t, r
+
+        k = 2 * args.cpu_threads
+        dtc, r = cpu(k)
+        k2 = int(min(n_gen * nfrag, k * args.cpu_seconds / max(dtc, 1e-3)))
+        if k2 > k:
+            k = k2
+            dtc, r = cpu(k)
+        res["cpu_baseline"] = {"value": r["n_candidates"] / dtc, "unit": "candidates/s", "cores": args.cpu_threads,
+                               "kind": "port",
+                               "sample": f"{k} selection tasks (generator, fragment of {nfrag}) over the leading "
+                                         f"generators of the same psi, reference selection restated in "
+                                         f"C/OpenMP; {r['n_candidates']} candidates in {dtc:.1f} s"}
+        log(f"bench: pt2 cpu oracle {res['cpu_baseline']['value']:.3e} candidates/s on {args.cpu_threads} threads")
+    ctx.close()
+    return res
 
 
 if __name__ == "__main__":

@@ -75,6 +75,8 @@ def lib():
         L.oq_sel_buffer_create.argtypes = [C.c_int, C.c_int, C.c_int]
         L.oq_sel_buffer_destroy.argtypes = [C.c_void_p]
         L.oq_select_all.argtypes = [C.c_void_p, C.c_void_p, c_dp, c_dp, C.c_void_p, C.c_int]
+        L.oq_select_tasks.argtypes = [C.c_void_p, C.c_void_p, c_dp, c_dp, C.c_void_p, C.c_int, C.c_int,
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.oq_select_connected.argtypes = [C.c_void_p, C.c_void_p, C.c_int, c_dp, c_dp, C.c_void_p, C.c_int]
         _LIB = L
     return _LIB
@@ -258,10 +260,12 @@ class _SelCtx(C.Structure):
 
 
 def selection(ham: Ham, dets_sorted, coefs_sorted, n_gen: int, n_sel: int, E0, n_keep: int,
-              nthreads: int = 0, block_log_cap: int = 0, block_keep=None, block_keep_offsets=None):
+              nthreads: int = 0, block_log_cap: int = 0, block_keep=None, block_keep_offsets=None, tasks=None):
     """ZMQ_selection on a psi already sorted by decreasing weight.
 
     dets_sorted (N_det, 2, n_int); coefs_sorted (N_det, N_states).
+    tasks: optional (n, 2) int array of (i_generator, subset) pairs, the
+    reference's selection tasks; default every generator whole.
     Returns dict(pt2, dets, vals, n_candidates, n_gathers, n_blocks, block_log)."""
     dets = _dets(dets_sorted)
     n_det, _, nint = dets.shape
@@ -303,7 +307,15 @@ def selection(ham: Ham, dets_sorted, coefs_sorted, n_gen: int, n_sel: int, E0, n
     E0 = np.ascontiguousarray(np.atleast_1d(np.asarray(E0, dtype=np.float64)))
     pt2 = np.zeros(ns)
     buf = lib().oq_sel_buffer_create(n_keep, 2 * n_keep, nint)
-    lib().oq_select_all(ham.ptr, C.byref(ctx), _dp(E0), _dp(pt2), buf, nthreads)
+    if tasks is None:
+        lib().oq_select_all(ham.ptr, C.byref(ctx), _dp(E0), _dp(pt2), buf, nthreads)
+    else:
+        tk = np.asarray(tasks, dtype=np.int32).reshape(-1, 2)
+        tg = np.ascontiguousarray(tk[:, 0])
+        tsb = np.ascontiguousarray(tk[:, 1])
+        ip = C.POINTER(C.c_int)
+        lib().oq_select_tasks(ham.ptr, C.byref(ctx), _dp(E0), _dp(pt2), buf, nthreads, tk.shape[0],
+                              tg.ctypes.data_as(ip), tsb.ctypes.data_as(ip))
     b = _SelBuf.from_address(buf)
     cur = b.cur
     out_d = np.ctypeslib.as_array(b.det, shape=(b.cap * 2 * nint,))[: cur * 2 * nint].reshape(cur, 2, nint).copy()

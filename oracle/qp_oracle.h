@@ -160,6 +160,8 @@ void oq_select_connected(const oq_ham *H, oq_sel_ctx *S, int i_generator, const 
  * per-thread buffers merged as the collector does). */
 void oq_select_all(const oq_ham *H, oq_sel_ctx *S, const double *E0, double *pt2,
                    oq_sel_buffer *out, int nthreads);
+void oq_select_tasks(const oq_ham *H, oq_sel_ctx *S, const double *E0, double *pt2, oq_sel_buffer *out,
+                     int nthreads, int n_tasks, const int *task_gen, const int *task_subset);
 
 #ifdef __cplusplus
 }

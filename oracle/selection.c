@@ -736,6 +736,15 @@ void oq_select_connected(const oq_ham *H, oq_sel_ctx *S, int i_generator, const 
 
 void oq_select_all(const oq_ham *H, oq_sel_ctx *S, const double *E0, double *pt2, oq_sel_buffer *out,
                    int nthreads) {
+  oq_select_tasks(H, S, E0, pt2, out, nthreads, 0, NULL, NULL);
+}
+
+/* The task farm of ZMQ_selection (zmq_selection.irp.f:20-40): one task is a
+   (subset, i_generator) pair; subset 0 = every block of the generator,
+   subset k = the blocks with maskInd mod fragment_count == k-1
+   (selection.irp.f:561-562).  n_tasks == 0 runs every generator whole. */
+void oq_select_tasks(const oq_ham *H, oq_sel_ctx *S, const double *E0, double *pt2, oq_sel_buffer *out,
+                     int nthreads, int n_tasks, const int *task_gen, const int *task_subset) {
   int NS = S->n_states, nint = H->nint;
   if (nthreads <= 0) nthreads = omp_get_max_threads();
   for (int st = 0; st < NS; ++st) pt2[st] = 0.0;
@@ -746,8 +755,14 @@ void oq_select_all(const oq_ham *H, oq_sel_ctx *S, const double *E0, double *pt2
   {
     int t = omp_get_thread_num();
     bufs[t] = oq_sel_buffer_create(N, 2 * N, nint);
+    if (n_tasks > 0) {
 #pragma omp for schedule(dynamic, 1)
-    for (int g = 1; g <= S->n_gen; ++g) oq_select_connected(H, S, g, E0, pt2t + (int64_t)t * NS, bufs[t], 0);
+      for (int k = 0; k < n_tasks; ++k)
+        oq_select_connected(H, S, task_gen[k], E0, pt2t + (int64_t)t * NS, bufs[t], task_subset[k]);
+    } else {
+#pragma omp for schedule(dynamic, 1)
+      for (int g = 1; g <= S->n_gen; ++g) oq_select_connected(H, S, g, E0, pt2t + (int64_t)t * NS, bufs[t], 0);
+    }
     oq_sel_buffer_sort(bufs[t]);
   }
   /* collector: gather every slave buffer and keep the N smallest values */

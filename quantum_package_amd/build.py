@@ -42,6 +42,24 @@ def _compile(src: str) -> str:
     return obj
 
 
+FLANG = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+FORTRAN_DRIVER = os.path.join(HERE, "qpx_sigma_driver")
+
+
+def build_fortran() -> str:
+    """The Fortran binding module and its example host program (links libqpx)."""
+    fdir = os.path.join(OBJ, "fortran")
+    os.makedirs(fdir, exist_ok=True)
+    src = os.path.join(HERE, "fortran")
+    subprocess.check_call([FLANG, "-O2", "-J", fdir, "-c", os.path.join(src, "qpx_mod.f90"), "-o",
+                           os.path.join(fdir, "qpx_mod.o")])
+    subprocess.check_call([FLANG, "-O2", "-J", fdir, "-c", os.path.join(src, "qpx_sigma_driver.f90"), "-o",
+                           os.path.join(fdir, "driver.o")])
+    subprocess.check_call([FLANG, "-o", FORTRAN_DRIVER, os.path.join(fdir, "driver.o"), os.path.join(fdir, "qpx_mod.o"),
+                           "-L" + HERE, "-lqpx", "-Wl,-rpath,$ORIGIN"])
+    return FORTRAN_DRIVER
+
+
 def build(jobs: int = 6) -> str:
     os.makedirs(OBJ, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
@@ -51,6 +69,8 @@ def build(jobs: int = 6) -> str:
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-o", tmp]
     subprocess.check_call(cmd)
     os.replace(tmp, OUT)
+    if os.path.exists(FLANG):
+        build_fortran()
     return OUT
 
 

@@ -32,8 +32,8 @@ def _hf_guess(dets, M, na, nb, n_st=1, seed=17):
 
 
 @pytest.mark.parametrize("name,M,na,nb,n,seed,n_st", [
-    ("config1", 8, 4, 4, None, 1001, 1),
-    ("config1-3roots", 8, 4, 4, None, 1001, 3),
+    ("fci-m8-4a4b", 8, 4, 4, None, 1001, 1),
+    ("fci-m8-4a4b-3roots", 8, 4, 4, None, 1001, 3),
     ("sparse-M12", 12, 4, 4, 3000, 7, 2),
     ("nint2-M70", 70, 3, 3, 2500, 8, 1),
 ])

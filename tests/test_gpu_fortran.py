@@ -1,0 +1,55 @@
+"""The Fortran binding (quantum_package_amd/fortran/qpx_mod.f90): a plain
+Fortran host program calls qpx_H_S2_u_0_nstates and qpx_davidson_diag with
+the reference's array layouts; results are checked against the oracle."""
+from __future__ import annotations
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from quantum_package_amd.hamiltonian import fci_space, synthetic_hamiltonian
+from oracle import oracle as O
+from test_gpu_sigma import _say
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "quantum_package_amd", "qpx_sigma_driver")
+
+
+def test_fortran_host_sigma_and_davidson(tmp_path):
+    if not os.path.exists(DRIVER):
+        pytest.skip("Fortran driver not built (amdflang missing at build time)")
+    H = synthetic_hamiltonian(8, 4, 4, seed=1001)
+    dets = fci_space(8, 4, 4)
+    n = dets.shape[0]
+    rng = np.random.default_rng(2)
+    u = rng.standard_normal((n, 2))
+    u[:, 0] = 0.0
+    hf = np.where((dets[:, 0, 0] == 15) & (dets[:, 1, 0] == 15))[0][0]
+    u[hf, 0] = 1.0
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        np.array([8, 4, 4, 1], dtype=np.int32).tofile(f)
+        np.array([n], dtype=np.int64).tofile(f)
+        np.array([2], dtype=np.int32).tofile(f)
+        np.array([H.eri.size], dtype=np.int64).tofile(f)
+        np.asfortranarray(H.h).ravel(order="F").tofile(f)
+        H.eri.tofile(f)
+        np.ascontiguousarray(dets).tofile(f)          # det(N_int,2,N_det) Fortran order
+        u.ravel(order="F").tofile(f)
+    _say("[fortran] qpx_sigma_driver on FCI M=8 4a4b")
+    r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = np.fromfile(fout, dtype=np.float64)
+    v = out[: 2 * n].reshape((n, 2), order="F")
+    s = out[2 * n: 4 * n].reshape((n, 2), order="F")
+    e = out[4 * n]
+    ham = O.Ham(H)
+    bil = O.Bilinear(dets)
+    vo, so = O.sigma(ham, bil, u)
+    assert np.max(np.abs(v - vo)) < 1e-11
+    assert np.max(np.abs(s - so)) < 1e-11
+    eo, _, _, _ = O.davidson(ham, bil, dets, u[:, 0], n_st=1, n_st_diag=4, threshold=1e-10)
+    assert abs(e - eo[0]) < 1e-9

@@ -53,7 +53,7 @@ def _random_space(M, na, nb, n, seed):
 
 
 CASES = [
-    ("config1-fci", 8, 4, 4, None, 1001),
+    ("fci-m8-4a4b", 8, 4, 4, None, 1001),
     ("fci-M10-3a3b", 10, 3, 3, None, 5),
     ("fci-M9-4a3b", 9, 4, 3, None, 6),
     ("sparse-M12", 12, 4, 4, 3000, 7),

@@ -202,7 +202,7 @@ def test_sigma_sparse_subset():
         assert np.max(np.abs(v[:, k] - want)) < 1e-11
 
 
-def test_davidson_config1_lanczos():
+def test_davidson_fci_m8_lanczos():
     from scipy.sparse.linalg import LinearOperator, eigsh
     H = synthetic_hamiltonian(8, 4, 4, seed=1001)
     kh, a, b = _kh(H)
