@@ -291,6 +291,12 @@ def pt2_leg(args, log):
            "workload": f"selection + EN-PT2, synthetic Hamiltonian seed {seed}: {M} MOs, {na} alpha + {nb} beta "
                        f"electrons, N_int=2; psi = {n} determinants (HF + one selection step, diagonalised); the "
                        f"{n_gen} leading generators by weight against all {n} selectors, to_select = N_det"}
+    # algorithmic bytes of the selection: 8 per integral/coefficient gather,
+    # (8 N_states + 16) per evaluated candidate, 8 N_states M^2 per block tile
+    b_alg = 8 * st["gathers"] + st["candidates"] * (8 * 1 + 16) + st["blocks"] * 8 * 1 * M * M
+    res["roofline"] = {"bound": "hbm", "achieved": b_alg / dt / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                       "frac": b_alg / dt / HBM_PEAK, "alg_bytes": b_alg,
+                       "note": "whole qpx_select call (host block enumeration and final sort included)"}
     log(f"bench: pt2 {n_gen} generators: {dt:.2f} s, {st['candidates']} candidates, "
         f"{res['value']:.3e} candidates/s")
     if not args.no_cpu:

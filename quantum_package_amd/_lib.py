@@ -14,7 +14,7 @@ c_i64p = C.POINTER(C.c_int64)
 c_ip = C.POINTER(C.c_int)
 c_vp = C.c_void_p
 
-# (name, restype, argtypes) of every entry point declared in include/qpx.h
+# (name, restype, argtypes) of every entry point declared in include/*.h
 SIGNATURES = [
     ("qpx_create", C.c_int, [C.c_int, C.POINTER(c_vp)]),
     ("qpx_destroy", C.c_int, [c_vp]),
@@ -30,13 +30,7 @@ SIGNATURES = [
     ("qpx_select", C.c_int, [c_vp, C.c_int, c_dp, C.c_int, C.c_int, c_dp, C.c_int64, C.c_int, c_dp, c_u64p,
                              c_dp, c_i64p]),
     ("qpx_select_stats", C.c_int, [c_vp, c_i64p]),
-]
-
-# internal hooks used by the tests (not part of the C-ABI header)
-EXTRA = [
-    ("qpx_debug_force_sigma_path", C.c_int, [c_vp, C.c_int, c_ip]),
-    ("qpx_debug_sigma_kernel_times", C.c_int, [c_vp, C.c_int, C.c_int, c_vp, c_vp, c_vp, c_vp,
-                                               C.POINTER(C.c_float), C.c_int]),
+    # include/qpx_proposal.h
     ("qpx_select_blocks", C.c_int, [c_vp, C.c_int, c_dp, C.c_int, C.c_int, c_dp, C.c_int64,
                                     C.POINTER(C.c_uint8), c_dp, c_u64p, c_dp, c_i64p, c_dp, c_i64p]),
     ("qpx_select_count_blocks", C.c_int, [c_vp, C.c_int, c_i64p]),
@@ -49,6 +43,13 @@ EXTRA = [
     ("qpx_mlp_param", C.c_int, [c_vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int64]),
     ("qpx_mlp_gemm_check", C.c_int, [c_vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                      C.POINTER(C.c_float)]),
+]
+
+# internal hooks used by the tests (not part of the C-ABI header)
+EXTRA = [
+    ("qpx_debug_force_sigma_path", C.c_int, [c_vp, C.c_int, c_ip]),
+    ("qpx_debug_sigma_kernel_times", C.c_int, [c_vp, C.c_int, C.c_int, c_vp, c_vp, c_vp, c_vp,
+                                               C.POINTER(C.c_float), C.c_int]),
 ]
 
 _LIB = None

@@ -18,6 +18,12 @@ Per iteration:
      sharing an occupation pattern with psi is added
      (src/Determinants/occ_pattern.irp.f:251-309).
 
+With a proposer (quantum_package_amd.proposal.Proposer) step 3 enumerates
+only the (generator, hole-pair) blocks the network proposes; every
+full_sweep_every-th iteration (and until the network is trained) the full
+enumeration runs instead, trains the network on its exact block PT2 and
+records the PT2 fraction the proposal would have missed.
+
 The backend is libqpx (quantum_package_amd.api.Context); the same loop runs
 on any object exposing set_wavefunction / davidson_diag_hs2 / ZMQ_selection.
 """
@@ -138,7 +144,7 @@ class CipsiResult:
 def cipsi(ctx, H, n_det_max: int, n_states: int = 1, n_states_diag: int = 4, s2_eig: bool = True,
           threshold_davidson: float = 1e-10, threshold_generators: float = 0.99,
           threshold_selectors: float = 0.999, pt2_max: float = 1e-4, guess_dets=None, log=print,
-          mixed: bool = False):
+          mixed: bool = False, proposer=None, full_sweep_every: int = 2):
     """Run the CIPSI loop on the GPU context ctx from HF (or guess_dets)."""
     from .hamiltonian import hf_determinant
     M, na, nb = H.M, H.n_alpha, H.n_beta
@@ -191,7 +197,11 @@ def cipsi(ctx, H, n_det_max: int, n_states: int = 1, n_states_diag: int = 4, s2_
         to_select = (n_det // 2 + 1) if s2_eig else n_det
         to_select = min(to_select, n_det_max - n_det)
         t1 = time.time()
-        pt2, new_dets, vals = ctx.ZMQ_selection(np.asfortranarray(coef_s), n_gen, n_sel, E0, to_select)
+        if proposer is None:
+            pt2, new_dets, vals = ctx.ZMQ_selection(np.asfortranarray(coef_s), n_gen, n_sel, E0, to_select)
+        else:
+            pt2, new_dets = _proposed_selection(ctx, H, proposer, dets_s, coef_s, n_gen, n_sel, E0, to_select,
+                                                len(history), full_sweep_every, rec, log)
         rec.update(pt2=pt2.tolist(), n_gen=n_gen, n_sel=n_sel, n_new=int(new_dets.shape[0]),
                    t_sel=time.time() - t1)
         # new determinants with zero coefficients, psi normalised
@@ -208,3 +218,33 @@ def cipsi(ctx, H, n_det_max: int, n_states: int = 1, n_states_diag: int = 4, s2_
         log(f"cipsi: N_det={n_det} E={energies} PT2={pt2} gen={n_gen} sel={n_sel} -> {dets.shape[0]} "
             f"({rec['t_diag']:.1f}s diag, {rec['t_sel']:.1f}s sel)")
     return CipsiResult(dets=dets, coef=coef, energies=energies, pt2=pt2, history=history)
+
+
+def _proposed_selection(ctx, H, proposer, dets_s, coef_s, n_gen, n_sel, E0, to_select, it, full_every, rec, log):
+    """Selection restricted to the proposed blocks, or a full sweep that
+    trains the proposer and measures what it would have missed.  A proposal
+    that cannot supply to_select determinants falls back to the full sweep."""
+    from .proposal import block_descriptors
+    desc, fdesc = block_descriptors(H, dets_s, coef_s, n_gen)
+    cf = np.asfortranarray(coef_s)
+    if proposer.ready and it % full_every != 0:
+        keep = proposer.propose(desc, fdesc)
+        pt2, new_dets, _, _, _ = ctx.select_blocks(cf, n_gen, n_sel, E0, to_select, keep=keep)
+        if new_dets.shape[0] >= to_select:
+            rec.update(sweep="proposed", kept=float(keep.mean()))
+            return pt2, new_dets
+        rec["fallback"] = True
+    pt2, new_dets, _, bpt2, _ = ctx.select_blocks(cf, n_gen, n_sel, E0, to_select)
+    rec["sweep"] = "full"
+    if proposer.ready:
+        keep = proposer.propose(desc, fdesc)
+        tot = np.abs(bpt2[:, 0]).sum()
+        rec["missed_pt2_fraction"] = float(1.0 - np.abs(bpt2[keep == 1, 0]).sum() / max(tot, 1e-300))
+        rec["would_keep"] = float(keep.mean())
+    t = time.time()
+    if proposer.fit(desc, fdesc, bpt2):
+        rec["t_train"] = time.time() - t
+        log(f"cipsi: full sweep, proposer trained on {desc.shape[0]} blocks ({rec['t_train']:.1f}s)"
+            + (f"; it would have missed {rec['missed_pt2_fraction']:.2e} of PT2 keeping {rec['would_keep']:.3f}"
+               if "missed_pt2_fraction" in rec else ""))
+    return pt2, new_dets

@@ -121,11 +121,12 @@ def synthetic_hamiltonian(M: int, n_alpha: int, n_beta: int, seed: int) -> Hamil
     Bp = B[:, pp - 1, qq - 1]                      # (nL, npair)
     delta = (pp == qq).astype(np.float64)
     V = Bp.T @ Bp + 0.3 * np.outer(delta, delta)    # (npair, npair) = (PQ|RS)
-    # packed: key = tri(P, R) for P <= R -> R(R-1)/2 + P
-    iu_r, iu_c = np.triu_indices(npair)             # row <= col
-    keys = iu_c * (iu_c + 1) // 2 + iu_r            # 0-based: tri(P,R)-1 with P=row+1, R=col+1
+    # packed: key = tri(P, R) for P <= R -> R(R-1)/2 + P, so for each R the
+    # keys of P = 1..R are contiguous (V is symmetric: row R-1 = column R-1)
     eri = np.empty(n_packed(M), dtype=np.float64)
-    eri[keys] = V[iu_r, iu_c]
+    for c in range(npair):
+        o = c * (c + 1) // 2
+        eri[o:o + c + 1] = V[c, :c + 1]
     return Hamiltonian(M=M, n_alpha=n_alpha, n_beta=n_beta, h=np.ascontiguousarray(h), eri=eri, seed=seed)
 
 

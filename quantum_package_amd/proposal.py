@@ -13,7 +13,8 @@ from __future__ import annotations
 
 import numpy as np
 
-LOG_FLOOR = float(np.log(1e-14))
+LABEL_SPAN = 15.0
+CALIBRATION_TARGET = 5e-4
 
 
 def block_order(na: int, nb: int):
@@ -93,9 +94,12 @@ def block_descriptors(H, dets_sorted: np.ndarray, coef_sorted: np.ndarray, n_gen
     return desc, fdesc
 
 
-def labels_from_blocks(block_pt2: np.ndarray) -> np.ndarray:
-    """log|e| of each block (state 0), floored"""
-    return np.maximum(np.log(np.abs(block_pt2[:, 0]) + 1e-300), LOG_FLOOR).astype(np.float32)
+def labels_from_blocks(block_pt2: np.ndarray, span: float = LABEL_SPAN) -> np.ndarray:
+    """log|e| of each block (state 0), floored at max - span: blocks more
+    than e^span below the largest carry no PT2 that matters, and the floor
+    keeps the regression on the range where the ranking is decided."""
+    y = np.log(np.abs(block_pt2[:, 0]) + 1e-300)
+    return np.maximum(y, y.max() - span).astype(np.float32)
 
 
 class Trainer:
@@ -136,14 +140,48 @@ class Trainer:
         return self.mlp.predict(desc, self._norm_f(fdesc)) * self.sd + self.mu
 
 
-def choose_blocks(pred_log: np.ndarray, drop_fraction: float = 2e-3):
-    """keep the blocks with the largest predicted |e| until the predicted
-    contribution of the dropped blocks is <= drop_fraction of the total"""
-    w = np.exp(pred_log.astype(np.float64))
-    order = np.argsort(w)  # ascending
-    cum = np.cumsum(w[order])
-    tot = cum[-1]
-    n_drop = int(np.searchsorted(cum, drop_fraction * tot, side="right"))
-    keep = np.ones(pred_log.size, dtype=np.uint8)
-    keep[order[:n_drop]] = 0
-    return keep
+def calibrate_threshold(pred_log: np.ndarray, block_pt2: np.ndarray, target: float = CALIBRATION_TARGET) -> float:
+    """Threshold tau on the predicted log|e| of a fully swept iteration such
+    that the blocks predicted below tau carry at most `target` of its exact
+    PT2.  Applied to the next iterations it misses several times `target`
+    (psi moves on), hence a target well below the allowed missed fraction."""
+    e = np.abs(block_pt2[:, 0])
+    order = np.argsort(pred_log, kind="stable")
+    cum = np.cumsum(e[order]) / max(e.sum(), 1e-300)
+    j = int(np.searchsorted(cum, target, side="right"))
+    return float(pred_log[order[j]]) if j < order.size else float("inf")
+
+
+def choose_blocks(pred_log: np.ndarray, tau: float) -> np.ndarray:
+    """keep mask (uint8, one per block): predicted log|e| >= tau"""
+    return (pred_log >= tau).astype(np.uint8)
+
+
+class Proposer:
+    """Train on a full sweep (exact block PT2), propose blocks afterwards."""
+
+    def __init__(self, mlp, epochs: int = 30, target: float = CALIBRATION_TARGET, lr: float = 1e-3,
+                 batch: int = 2048, seed: int = 0, min_blocks: int = 4096):
+        self.trainer = Trainer(mlp, lr=lr, batch=batch, seed=seed)
+        self.epochs = epochs
+        self.target = target
+        self.min_blocks = min_blocks
+        self.tau = None
+
+    @property
+    def ready(self) -> bool:
+        return self.tau is not None
+
+    def fit(self, desc, fdesc, block_pt2, log=None) -> bool:
+        """Train on one full sweep; sweeps smaller than min_blocks are too
+        small to learn from (and to standardise the labels on) and are
+        skipped.  Returns whether it trained."""
+        if desc.shape[0] < self.min_blocks:
+            return False
+        y = labels_from_blocks(block_pt2)
+        self.trainer.fit(desc, fdesc, y, epochs=self.epochs, log=log)
+        self.tau = calibrate_threshold(self.trainer.predict(desc, fdesc), block_pt2, self.target)
+        return True
+
+    def propose(self, desc, fdesc) -> np.ndarray:
+        return choose_blocks(self.trainer.predict(desc, fdesc), self.tau)

@@ -1,6 +1,6 @@
 """CPU-side checks of the drop-in boundary (no GPU needed).
 
-* libqpx.so loads and exports every entry point declared in include/qpx.h;
+* libqpx.so loads and exports every entry point declared in include/*.h;
 * the Python mirror binds every declared entry point;
 * the synthetic Hamiltonian generator and determinant helpers (host logic).
 """
@@ -16,12 +16,12 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "qpx.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("qpx.h", "qpx_proposal.h")]
 LIB = os.path.join(ROOT, "quantum_package_amd", "libqpx.so")
 
 
 def declared_functions():
-    txt = open(HEADER).read()
+    txt = "".join(open(h).read() for h in HEADERS)
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\bint\s+(qpx_\w+)\s*\(", txt)))
 
