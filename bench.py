@@ -13,8 +13,8 @@ The JSON line also carries:
                 H_S2_u_0_nstates_openmp_work, the reference itself cannot be
                 built here) timed on a bounded, strided sample of the same
                 rows on this host's cores;
-  davidson      davidson_diag_hs2 on the same space (3 roots, 4 vectors,
-                threshold 1e-10): plain FP64 against mixed precision (FP32
+  davidson      davidson_diag_hs2 on the same space (lowest 3 roots, 4
+                vectors, threshold 1e-10): plain FP64 against mixed precision (FP32
                 correction-vector sigma, FP64 everything else), wall time,
                 sigma calls and |dE| per root;
   pt2           CIPSI selection + EN-PT2 candidates/s (FP64) on the
@@ -224,9 +224,11 @@ def davidson_leg(ctx, H, dets, log):
     g[i_hf, 0] = 1.0
     g[:, 1:] = np.random.default_rng(1002).standard_normal((n, 2))
     res = {}
+    ctx.set_verbosity(1)   # iteration table on stderr
     for name, mixed in (("fp64", False), ("mixed", True)):
+        log(f"bench: davidson {name} ...")
         t0 = time.time()
-        e, s2, u, ns = ctx.davidson_diag_hs2(g, N_st=3, N_st_diag=4, threshold=1e-10, s2_eig=True, mixed=mixed,
+        e, s2, u, ns = ctx.davidson_diag_hs2(g, N_st=3, N_st_diag=4, threshold=1e-10, s2_eig=False, mixed=mixed,
                                              switch_threshold=1e-6)
         dt = time.time() - t0
         res[name] = {"seconds": dt, "energies": e[:3].tolist(), "s2": s2[:3].tolist(),
@@ -235,8 +237,9 @@ def davidson_leg(ctx, H, dets, log):
     d = np.abs(np.array(res["mixed"]["energies"]) - np.array(res["fp64"]["energies"]))
     res["abs_dE_per_root"] = d.tolist()
     res["speedup_mixed"] = res["fp64"]["seconds"] / res["mixed"]["seconds"]
-    res["workload"] = ("davidson_diag_hs2, 3 roots + 1 extra vector, threshold 1e-10, s2_eig, HF + seeded random "
-                       "guess, same 9,018,009-determinant space")
+    res["workload"] = ("davidson_diag_hs2, lowest 3 roots of H (no S^2 filter: seeded random guess vectors are "
+                       "not spin-adapted) + 1 extra vector, threshold 1e-10 on the squared residual, HF + seeded "
+                       "random guess, same 9,018,009-determinant space")
     return res
 
 
@@ -281,9 +284,11 @@ def pt2_leg(args, log):
         pt2, sel, _ = ctx.ZMQ_selection(cs, n_gen, n, E0, n)
         return time.time() - t, pt2, ctx.select_stats(), sel.shape[0]
 
-    dt, _, st, _ = run(min(n, 64))          # warm-up and calibration
-    rate_gen = min(n, 64) / max(dt, 1e-3)
-    n_gen = int(min(n, max(64, rate_gen * args.pt2_seconds)))
+    log("bench: pt2 calibration on 8 generators ...")
+    dt, _, st, _ = run(min(n, 8))           # warm-up and calibration
+    rate_gen = min(n, 8) / max(dt, 1e-3)
+    n_gen = int(min(n, max(8, rate_gen * args.pt2_seconds)))
+    log(f"bench: pt2 {rate_gen:.1f} generators/s; timing {n_gen} generators ...")
     dt, pt2, st, n_new = run(n_gen)
     res = {"value": st["candidates"] / dt, "unit": "candidates/s", "dtype": "f64", "seconds": dt,
            "generators": n_gen, "selectors": n, "candidates": st["candidates"], "blocks": st["blocks"],

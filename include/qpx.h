@@ -36,6 +36,11 @@ int qpx_create(int device, qpx_ctx **ctx);
 int qpx_destroy(qpx_ctx *ctx);
 /* Copies the last error message (NUL-terminated) into buf. */
 int qpx_last_error(qpx_ctx *ctx, char *buf, int len);
+/* level 1: qpx_davidson writes one line per iteration (iteration, energy,
+ * S^2 and squared residual of each state) to stderr, as the reference
+ * writes its iteration table (src/Davidson/diagonalization_hs2.irp.f:402).
+ * 0 (default): silent. */
+int qpx_set_verbosity(qpx_ctx *ctx, int level);
 
 /* Hamiltonian: h_core (M x M, column-major) = mo_mono_elec_integral
  * (src/Integrals_Monoelec/mo_mono_ints.irp.f:1-25); eri_packed (n8) holds
@@ -84,7 +89,10 @@ int qpx_diag_h(qpx_ctx *ctx, double *h_jj);
  * sigma for the correction vectors until the squared residual falls below
  * switch_threshold, then FP64 sigma (Rayleigh quotient, orthogonalisation,
  * residuals always FP64).  n_sigma_out (may be NULL) receives
- * {number of FP64 sigma calls, number of FP32 sigma calls}. */
+ * {number of FP64 sigma calls, number of FP32 sigma calls}.  The
+ * reference iterates until converged; here the iteration count is capped at
+ * 1000 (an error), so a threshold below the attainable precision fails
+ * instead of looping. */
 int qpx_davidson(qpx_ctx *ctx, int n_st, int n_st_diag, double threshold, int sze_max, int s2_eig,
                  int state_following, int mixed, double switch_threshold, uint64_t seed, double *u_in,
                  int64_t dim_in, double *energies, double *s2_out, int *n_sigma_out);

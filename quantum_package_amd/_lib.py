@@ -19,6 +19,7 @@ SIGNATURES = [
     ("qpx_create", C.c_int, [C.c_int, C.POINTER(c_vp)]),
     ("qpx_destroy", C.c_int, [c_vp]),
     ("qpx_last_error", C.c_int, [c_vp, C.c_char_p, C.c_int]),
+    ("qpx_set_verbosity", C.c_int, [c_vp, C.c_int]),
     ("qpx_set_hamiltonian", C.c_int, [c_vp, C.c_int, C.c_int, C.c_int, c_dp, c_dp, C.c_int64]),
     ("qpx_set_wavefunction", C.c_int, [c_vp, C.c_int, C.c_int64, c_u64p]),
     ("qpx_sigma", C.c_int, [c_vp, C.c_int, C.c_int, C.c_int64, c_dp, c_dp, c_dp]),

@@ -58,6 +58,10 @@ class Context:
             pass
 
     # ---------------------------------------------------------------- inputs
+    def set_verbosity(self, level: int):
+        """1: print the Davidson iteration table to stderr."""
+        self._check(self.L.qpx_set_verbosity(self.ptr, int(level)))
+
     def set_hamiltonian(self, H):
         h = np.asfortranarray(H.h, dtype=np.float64)
         eri = np.ascontiguousarray(H.eri, dtype=np.float64)

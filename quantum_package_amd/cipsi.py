@@ -144,7 +144,7 @@ class CipsiResult:
 def cipsi(ctx, H, n_det_max: int, n_states: int = 1, n_states_diag: int = 4, s2_eig: bool = True,
           threshold_davidson: float = 1e-10, threshold_generators: float = 0.99,
           threshold_selectors: float = 0.999, pt2_max: float = 1e-4, guess_dets=None, log=print,
-          mixed: bool = False, proposer=None, full_sweep_every: int = 2):
+          mixed: bool = False, proposer=None, full_sweep_every: int = 2, audit_proposal: bool = False):
     """Run the CIPSI loop on the GPU context ctx from HF (or guess_dets)."""
     from .hamiltonian import hf_determinant
     M, na, nb = H.M, H.n_alpha, H.n_beta
@@ -201,7 +201,7 @@ def cipsi(ctx, H, n_det_max: int, n_states: int = 1, n_states_diag: int = 4, s2_
             pt2, new_dets, vals = ctx.ZMQ_selection(np.asfortranarray(coef_s), n_gen, n_sel, E0, to_select)
         else:
             pt2, new_dets = _proposed_selection(ctx, H, proposer, dets_s, coef_s, n_gen, n_sel, E0, to_select,
-                                                len(history), full_sweep_every, rec, log)
+                                                len(history), full_sweep_every, rec, log, audit_proposal)
         rec.update(pt2=pt2.tolist(), n_gen=n_gen, n_sel=n_sel, n_new=int(new_dets.shape[0]),
                    t_sel=time.time() - t1)
         # new determinants with zero coefficients, psi normalised
@@ -220,10 +220,13 @@ def cipsi(ctx, H, n_det_max: int, n_states: int = 1, n_states_diag: int = 4, s2_
     return CipsiResult(dets=dets, coef=coef, energies=energies, pt2=pt2, history=history)
 
 
-def _proposed_selection(ctx, H, proposer, dets_s, coef_s, n_gen, n_sel, E0, to_select, it, full_every, rec, log):
+def _proposed_selection(ctx, H, proposer, dets_s, coef_s, n_gen, n_sel, E0, to_select, it, full_every, rec, log,
+                        audit=False):
     """Selection restricted to the proposed blocks, or a full sweep that
     trains the proposer and measures what it would have missed.  A proposal
-    that cannot supply to_select determinants falls back to the full sweep."""
+    that cannot supply to_select determinants falls back to the full sweep.
+    audit: also run the full sweep on proposed iterations and record the
+    PT2 fraction the proposal actually missed (for tests and measurement)."""
     from .proposal import block_descriptors
     desc, fdesc = block_descriptors(H, dets_s, coef_s, n_gen)
     cf = np.asfortranarray(coef_s)
@@ -232,6 +235,9 @@ def _proposed_selection(ctx, H, proposer, dets_s, coef_s, n_gen, n_sel, E0, to_s
         pt2, new_dets, _, _, _ = ctx.select_blocks(cf, n_gen, n_sel, E0, to_select, keep=keep)
         if new_dets.shape[0] >= to_select:
             rec.update(sweep="proposed", kept=float(keep.mean()))
+            if audit:
+                pt2_full, _, _ = ctx.ZMQ_selection(cf, n_gen, n_sel, E0, to_select)
+                rec["missed_pt2_fraction"] = float(1.0 - pt2[0] / pt2_full[0])
             return pt2, new_dets
         rec["fallback"] = True
     pt2, new_dets, _, bpt2, _ = ctx.select_blocks(cf, n_gen, n_sel, E0, to_select)
@@ -239,12 +245,12 @@ def _proposed_selection(ctx, H, proposer, dets_s, coef_s, n_gen, n_sel, E0, to_s
     if proposer.ready:
         keep = proposer.propose(desc, fdesc)
         tot = np.abs(bpt2[:, 0]).sum()
-        rec["missed_pt2_fraction"] = float(1.0 - np.abs(bpt2[keep == 1, 0]).sum() / max(tot, 1e-300))
+        rec["missed_pt2_fraction_if_kept"] = float(1.0 - np.abs(bpt2[keep == 1, 0]).sum() / max(tot, 1e-300))
         rec["would_keep"] = float(keep.mean())
     t = time.time()
     if proposer.fit(desc, fdesc, bpt2):
         rec["t_train"] = time.time() - t
         log(f"cipsi: full sweep, proposer trained on {desc.shape[0]} blocks ({rec['t_train']:.1f}s)"
-            + (f"; it would have missed {rec['missed_pt2_fraction']:.2e} of PT2 keeping {rec['would_keep']:.3f}"
-               if "missed_pt2_fraction" in rec else ""))
+            + (f"; kept on, it would have missed {rec['missed_pt2_fraction_if_kept']:.2e} of PT2 keeping "
+               f"{rec['would_keep']:.3f}" if "missed_pt2_fraction_if_kept" in rec else ""))
     return pt2, new_dets

@@ -45,7 +45,6 @@ extern "C" int qpx_create(int device, qpx_ctx **out) {
     hipDeviceProp_t prop;
     QPX_HIP(hipGetDeviceProperties(&prop, device));
     c->lds_per_block = (int)prop.sharedMemPerBlock;
-    if (const char *e = std::getenv("QPX_SIGMA_BS")) c->sigma_bs = std::atoi(e);
     c->n_cu = prop.multiProcessorCount;
   });
   if (rc) {
@@ -69,6 +68,10 @@ extern "C" int qpx_destroy(qpx_ctx *c) {
     if (c->kev[i]) (void)hipEventDestroy(c->kev[i]);
   delete c;  // DBuf destructors free device memory
   return 0;
+}
+
+extern "C" int qpx_set_verbosity(qpx_ctx *c, int level) {
+  return qpx_guard(c, [&] { c->verbosity = level; });
 }
 
 extern "C" int qpx_last_error(qpx_ctx *c, char *buf, int len) {

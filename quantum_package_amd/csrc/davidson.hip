@@ -25,6 +25,7 @@
 // so the reported energies come from an all-FP64 subspace.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 #include "qpx_internal.h"
@@ -419,6 +420,8 @@ extern "C" int qpx_davidson(qpx_ctx *c, int N_st, int N_st_diag, double threshol
     int n64 = 0, n32 = 0;
     bool converged = false;
     int shift2 = 0;
+    int iter_total = 0;
+    constexpr int kMaxIterations = 1000;
     while (!converged) {
       QPX_HIP(hipMemcpyAsync(U.p, uin_d.p, sizeof(double) * n * N_st_diag, hipMemcpyDeviceToDevice, st));
       bool switched = false;
@@ -522,8 +525,16 @@ extern "C" int qpx_davidson(qpx_ctx *c, int N_st, int N_st_diag, double threshol
           resid[k] = r;
           if (k < N_st) rmax = std::max(rmax, r);
         }
+        ++iter_total;
+        if (c->verbosity > 0) {
+          std::fprintf(stderr, "davidson %4d %s", iter_total, use_fp32 ? "fp32" : "fp64");
+          for (int k = 0; k < N_st; ++k) std::fprintf(stderr, "  %18.10f %9.6f %10.3e", lambda[k], s2[k], resid[k]);
+          std::fprintf(stderr, "\n");
+          std::fflush(stderr);
+        }
         for (int k = 0; k < N_st; ++k)
           QPX_REQUIRE(resid[k] <= 1e8 && std::isfinite(resid[k]), "Davidson failed (residual > 1e8)");
+        QPX_REQUIRE(iter_total < kMaxIterations, "Davidson did not converge in 1000 iterations");
         if (use_fp32 && rmax < switch_threshold) {
           // restart on the Ritz vectors with FP64 sigma
           use_fp32 = false;

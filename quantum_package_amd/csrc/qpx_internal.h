@@ -133,7 +133,7 @@ struct qpx_ctx {
   int64_t sel_stats[4] = {0, 0, 0, 0};
   int sigma_path = 0;           // last sigma path: 1 = dense-map, 2 = generic
   int force_path = 0;           // 0 auto, 1 dense-map, 2 generic (tests)
-  int sigma_bs = 0;             // column-kernel workgroup size (0 = automatic)
+  int verbosity = 0;            // 1: Davidson iteration table on stderr
   hipEvent_t kev[3] = {nullptr, nullptr, nullptr};  // per-kernel timing (benchmark hook)
   bool ktiming = false;
 };

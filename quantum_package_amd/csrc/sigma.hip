@@ -66,54 +66,55 @@ struct DenseArgs {
   int n_a, n_b, M, npair;
 };
 
-// Same-spin partners of one string from its conn lists (singles and
-// doubles), 4-way unrolled so that the list loads of consecutive entries are
-// in flight together.  map is the LDS map of the current column/row, g the
-// other-spin Coulomb slice g(h,p) = sum_k (hp|kk).
+// Same-spin partners of the 64 determinants of a wavefront, 16 lanes per
+// determinant: in round r, lane group q = lane/16 takes the determinant of
+// lane 4r+q and walks its singles and doubles lists with stride 16, so the
+// list loads are coalesced instead of one list per lane.  The partial sums
+// are reduced over the 16 lanes and added to acc of the owning lane.  Every
+// lane of the wavefront must call it (lanes without a determinant pass
+// empty ranges).
 template <typename T, int NST>
-__device__ inline void same_spin_partners(const int *__restrict__ s_idx, const uint32_t *__restrict__ s_info,
-                                          const T *__restrict__ s_val, int s0, int s1,
-                                          const int *__restrict__ d_idx, const T *__restrict__ d_val, int d0,
-                                          int d1, const T *map, const T *g, int M, T *acc) {
-  int es = s0;
-  for (; es + 4 <= s1; es += 4) {
-    int ap[4];
-    uint32_t inf[4];
-    T sv[4];
+__device__ inline void same_spin_x16(const int *__restrict__ s_idx, const uint32_t *__restrict__ s_info,
+                                     const T *__restrict__ s_val, int s0, int s1, const int *__restrict__ d_idx,
+                                     const T *__restrict__ d_val, int d0, int d1, const T *map, const T *g, int M,
+                                     T *acc) {
+  const int lane = threadIdx.x & 63;
+  const int q = lane >> 4, sub = lane & 15;
+#pragma unroll 1
+  for (int r = 0; r < 16; ++r) {
+    const int src = 4 * r + q;
+    const int rs0 = __shfl(s0, src), rs1 = __shfl(s1, src), rd0 = __shfl(d0, src), rd1 = __shfl(d1, src);
+    T p[NST];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) { ap[q] = s_idx[es + q]; inf[q] = s_info[es + q]; sv[q] = s_val[es + q]; }
+    for (int st = 0; st < NST; ++st) p[st] = T(0);
+    for (int es = rs0 + sub; es < rs1; es += 16) {
+      const int ap = s_idx[es];
+      const uint32_t inf = s_info[es];
+      const T gg = g[(inf & 0xff) * M + ((inf >> 8) & 0xff)];
+      const T val = s_val[es] + (((inf >> 16) & 1) ? -gg : gg);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      T gg = g[(inf[q] & 0xff) * M + ((inf[q] >> 8) & 0xff)];
-      T val = sv[q] + (((inf[q] >> 16) & 1) ? -gg : gg);
-#pragma unroll
-      for (int st = 0; st < NST; ++st) acc[st] += val * map[ap[q] * NST + st];
+      for (int st = 0; st < NST; ++st) p[st] += val * map[ap * NST + st];
     }
-  }
-  for (; es < s1; ++es) {
-    const int ap = s_idx[es];
-    const uint32_t inf = s_info[es];
-    T gg = g[(inf & 0xff) * M + ((inf >> 8) & 0xff)];
-    T val = s_val[es] + (((inf >> 16) & 1) ? -gg : gg);
+    int ed = rd0 + sub;
+    for (; ed + 16 < rd1; ed += 32) {
+      const int ap0 = d_idx[ed], ap1 = d_idx[ed + 16];
+      const T v0 = d_val[ed], v1 = d_val[ed + 16];
 #pragma unroll
-    for (int st = 0; st < NST; ++st) acc[st] += val * map[ap * NST + st];
-  }
-  int ed = d0;
-  for (; ed + 4 <= d1; ed += 4) {
-    int ap[4];
-    T dv[4];
+      for (int st = 0; st < NST; ++st) p[st] += v0 * map[ap0 * NST + st] + v1 * map[ap1 * NST + st];
+    }
+    if (ed < rd1) {
+      const int ap0 = d_idx[ed];
+      const T v0 = d_val[ed];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) { ap[q] = d_idx[ed + q]; dv[q] = d_val[ed + q]; }
+      for (int st = 0; st < NST; ++st) p[st] += v0 * map[ap0 * NST + st];
+    }
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int st = 0; st < NST; ++st) {
 #pragma unroll
-      for (int st = 0; st < NST; ++st) acc[st] += dv[q] * map[ap[q] * NST + st];
-  }
-  for (; ed < d1; ++ed) {
-    const int ap = d_idx[ed];
-    const T dv = d_val[ed];
-#pragma unroll
-    for (int st = 0; st < NST; ++st) acc[st] += dv * map[ap * NST + st];
+      for (int off = 8; off > 0; off >>= 1) p[st] += __shfl_xor(p[st], off);
+      const T t = __shfl(p[st], (lane & 3) * 16);
+      if ((lane >> 2) == r) acc[st] += t;
+    }
   }
 }
 
@@ -137,20 +138,25 @@ __device__ inline void coulomb_slice(const uint64_t *sw, const T *__restrict__ V
   }
 }
 
-// Column kernel: workgroup = (beta column b, chunk of 1024 of its
+// Column kernel: workgroup = (beta column b, chunk of BS of its
 // determinants), one determinant per thread.  The thread keeps the
-// alpha-singles of its alpha string in registers (packed a' | h<<16 |
-// p<<23 | phase<<30) for the whole beta-single loop.
-// LDS layout: map[n_a * NST] | slice[M*M]
+// alpha-singles of its alpha string in registers, packed
+// a' | (h*M + p) << 16 | phase << 30, for the whole beta-single loop;
+// entries past its count point at a map slot that stays zero, so the inner
+// loop runs branch-free to a wavefront-uniform trip count.
+// LDS layout: map[(n_a + 1) * NST] (slot n_a = 0) | slice[M*M] |
+// sacc[BS * NST] (each thread's S^2 accumulators, off the register file)
 template <typename T, int NI, int NST, int MAXS, int BS>
 __global__ __launch_bounds__(BS) void sigma_dense_col_kernel(DenseArgs A, int nchunk, const T *__restrict__ V,
                                                                const T *__restrict__ as_val,
                                                                const T *__restrict__ ad_val,
                                                                const T *__restrict__ u, T *__restrict__ v,
                                                                T *__restrict__ s) {
+  static_assert(MAXS % 4 == 0, "alpha-singles are consumed in groups of 4");
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *map = reinterpret_cast<T *>(lds_raw);
-  T *slice = map + (size_t)A.n_a * NST;
+  T *slice = map + (size_t)(A.n_a + 1) * NST;
+  T *sacc = slice + (size_t)A.M * A.M + (size_t)threadIdx.x * NST;
   // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so
   // logical block = (id mod 8) * (grid/8) + id/8 keeps the chunks of one
   // column, and neighbouring columns, on one XCD and its L2
@@ -170,19 +176,25 @@ __global__ __launch_bounds__(BS) void sigma_dense_col_kernel(DenseArgs A, int nc
     sbeg = A.as_ptr[a];
     ns = A.as_ptr[a + 1] - sbeg;
   }
+  const uint32_t zero_slot = (uint32_t)A.n_a;
+  if (tid < NST) map[(size_t)A.n_a * NST + tid] = T(0);
   uint32_t L[MAXS];
 #pragma unroll
   for (int j = 0; j < MAXS; ++j) {
-    uint32_t x = 0;
+    uint32_t x = zero_slot;
     if (j < ns) {
       uint32_t inf = A.as_info[sbeg + j];
-      x = (uint32_t)A.as_idx[sbeg + j] | ((inf & 0x7f) << 16) | (((inf >> 8) & 0x7f) << 23) | (((inf >> 16) & 1) << 30);
+      x = (uint32_t)A.as_idx[sbeg + j] | (((inf & 0xff) * M + ((inf >> 8) & 0xff)) << 16) | (((inf >> 16) & 1) << 30);
     }
     L[j] = x;
   }
-  T acc[NST], accs[NST];
+  int nsw = ns;
 #pragma unroll
-  for (int st = 0; st < NST; ++st) acc[st] = accs[st] = T(0);
+  for (int off = 32; off > 0; off >>= 1) nsw = max(nsw, __shfl_xor(nsw, off));
+  nsw = __builtin_amdgcn_readfirstlane(nsw);
+  T acc[NST];
+#pragma unroll
+  for (int st = 0; st < NST; ++st) acc[st] = sacc[st] = T(0);
   uint64_t bw[NI];
 #pragma unroll
   for (int w = 0; w < NI; ++w) bw[w] = A.bstr[(int64_t)b * NI + w];
@@ -214,27 +226,46 @@ __global__ __launch_bounds__(BS) void sigma_dense_col_kernel(DenseArgs A, int nc
       slice[x] = V[pair_index(h, p) * A.npair + Pb];
     }
     __syncthreads();
-    // binfo has no bits above 16: xor-ing it in keeps the decode of L[j]
-    // inside this loop (hoisting it would multiply the live registers)
-    const uint32_t keep = binfo >> 24;
+    // binfo has no bits above 16, so keep is zero; xor-ing it in keeps the
+    // decode of L[j] inside this loop (hoisted, it would be spilled)
+    const uint32_t keep = (binfo >> 24) * 0x01010101u;
+    // the alpha single (pb -> hb) is the spin exchange of this beta single
+    // (S^2 off-diagonal); at most one j matches, remembered in fo
+    const uint32_t fidx = (uint32_t)(pb * M + hb);
+    uint32_t fo = zero_slot;
 #pragma unroll
-    for (int j = 0; j < MAXS; ++j) {
-      if (j < ns) {
-        const uint32_t x = L[j] ^ keep;
-        const int ap = x & 0xffff, ha = (x >> 16) & 0x7f, pa = (x >> 23) & 0x7f;
-        const bool neg = ((x >> 30) & 1) ^ phb;
-        T val = slice[ha * M + pa];
-        if (neg) val = -val;
-        const bool flip = (ha == pb) && (pa == hb);
+    for (int j0 = 0; j0 < MAXS; j0 += 4) {
+      if (j0 < nsw) {
+        T uu[4][NST], sv[4];
+        uint32_t xs[4];
 #pragma unroll
-        for (int st = 0; st < NST; ++st) {
-          T uu = map[ap * NST + st];
-          acc[st] += val * uu;
-          if (flip) accs[st] += neg ? uu : -uu;
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t x = L[j0 + q] ^ keep;
+          xs[q] = x;
+          const T *mp = map + (x & 0xffff) * NST;
+#pragma unroll
+          for (int st = 0; st < NST; ++st) uu[q][st] = mp[st];
+          sv[q] = slice[(x >> 16) & 0x3fff];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t negb = ((xs[q] >> 30) & 1) ^ (uint32_t)phb;
+          const T val = negb ? -sv[q] : sv[q];
+#pragma unroll
+          for (int st = 0; st < NST; ++st) acc[st] += val * uu[q][st];
+          fo = (((xs[q] >> 16) & 0x3fff) == fidx) ? ((xs[q] & 0xffff) | (negb << 31)) : fo;
         }
       }
-      // bound the number of LDS loads the scheduler keeps in flight
-      if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    {
+      const T *mp = map + (fo & 0xffff) * NST;
+      const bool fneg = (fo >> 31) != 0;
+#pragma unroll
+      for (int st = 0; st < NST; ++st) {
+        const T uu = mp[st];
+        sacc[st] += fneg ? uu : -uu;
+      }
     }
   }
 
@@ -251,15 +282,15 @@ __global__ __launch_bounds__(BS) void sigma_dense_col_kernel(DenseArgs A, int nc
   }
   coulomb_slice<T, NI>(bw, V, A.npair, M, slice);
   __syncthreads();
+  same_spin_x16<T, NST>(A.as_idx, A.as_info, as_val, sbeg, sbeg + ns, A.ad_idx, ad_val, mine ? A.ad_ptr[a] : 0,
+                        mine ? A.ad_ptr[a + 1] : 0, map, slice, M, acc);
   if (mine) {
-    same_spin_partners<T, NST>(A.as_idx, A.as_info, as_val, sbeg, sbeg + ns, A.ad_idx, ad_val, A.ad_ptr[a],
-                               A.ad_ptr[a + 1], map, slice, M, acc);
     const T hd = (T)A.hdiag[e], sd = (T)A.sdiag[e];
 #pragma unroll
     for (int st = 0; st < NST; ++st) {
       T uu = map[a * NST + st];
       v[(int64_t)e * NST + st] = acc[st] + hd * uu;
-      s[(int64_t)e * NST + st] = accs[st] + sd * uu;
+      s[(int64_t)e * NST + st] = sacc[st] + sd * uu;
     }
   }
 }
@@ -298,13 +329,15 @@ __global__ __launch_bounds__(1024) void sigma_dense_row_kernel(DenseArgs A, int 
   coulomb_slice<T, NI>(aw, V, A.npair, M, slice);
   __syncthreads();
   const int t = r0 + chunk * nth + tid;
-  if (t >= r1) return;
-  const int b = A.csr_b[t];
+  if (r0 + chunk * nth + (tid & ~63) >= r1) return;  // whole wavefront past the row
+  const bool mine = t < r1;
+  const int b = mine ? A.csr_b[t] : 0;
   T acc[NST];
 #pragma unroll
   for (int st = 0; st < NST; ++st) acc[st] = T(0);
-  same_spin_partners<T, NST>(A.bs_idx, A.bs_info, bs_val, A.bs_ptr[b], A.bs_ptr[b + 1], A.bd_idx, bd_val,
-                             A.bd_ptr[b], A.bd_ptr[b + 1], map, slice, M, acc);
+  same_spin_x16<T, NST>(A.bs_idx, A.bs_info, bs_val, mine ? A.bs_ptr[b] : 0, mine ? A.bs_ptr[b + 1] : 0, A.bd_idx,
+                        bd_val, mine ? A.bd_ptr[b] : 0, mine ? A.bd_ptr[b + 1] : 0, map, slice, M, acc);
+  if (!mine) return;
   const int e = A.csr2csc[t];
 #pragma unroll
   for (int st = 0; st < NST; ++st) v[(int64_t)e * NST + st] += acc[st];
@@ -542,7 +575,7 @@ static void launch_dense(qpx_ctx *c, const T *u, T *v, T *s, hipStream_t st) {
   DenseArgs A = dense_args(c);
   const Ham &H = c->H;
   const Wfn &W = c->W;
-  size_t lds_col = ((size_t)W.n_a * NST + (size_t)H.M * H.M) * sizeof(T);
+  size_t lds_col = ((size_t)(W.n_a + 1) * NST + (size_t)H.M * H.M + (size_t)BS * NST) * sizeof(T);
   size_t lds_row = ((size_t)W.n_b * NST + (size_t)H.M * H.M) * sizeof(T);
   auto kc = sigma_dense_col_kernel<T, NI, NST, MAXS, BS>;
   auto kr = sigma_dense_row_kernel<T, NI, NST>;
@@ -570,16 +603,11 @@ static void launch_dense_maxs_bs(qpx_ctx *c, int maxs, const T *u, T *v, T *s, h
   else launch_dense<T, NI, NST, 96, BS>(c, u, v, s, st);
 }
 
-// workgroup size of the column kernel: 1024 (16 waves, 128 VGPRs); 512 (8
-// waves, 256 VGPRs) avoids the few spills of long alpha-singles lists but
-// measured slower on the 14-orbital full-CI case (57.5 vs 45.4 ms);
-// QPX_SIGMA_BS overrides
+// workgroup size of the column kernel: 1024 (16 waves, 128 VGPRs; the
+// LDS of one column allows one workgroup per CU)
 template <typename T, int NI, int NST>
 static void launch_dense_maxs(qpx_ctx *c, int maxs, const T *u, T *v, T *s, hipStream_t st) {
-  int bs = c->sigma_bs;
-  if (bs == 0) bs = 1024;
-  if (bs == 512) launch_dense_maxs_bs<T, NI, NST, 512>(c, maxs, u, v, s, st);
-  else launch_dense_maxs_bs<T, NI, NST, 1024>(c, maxs, u, v, s, st);
+  launch_dense_maxs_bs<T, NI, NST, 1024>(c, maxs, u, v, s, st);
 }
 
 template <typename T, int NI>
@@ -630,8 +658,8 @@ static bool dense_ok(qpx_ctx *c, int nst, size_t tsize) {
   const Wfn &W = c->W;
   const Ham &H = c->H;
   if (W.nint > 2 || nst > 4) return false;
-  if (W.n_a > 65535 || H.M > 128 || W.ca.max_singles > 96) return false;
-  size_t need_c = ((size_t)W.n_a * nst + (size_t)H.M * H.M) * tsize;
+  if (W.n_a > 65534 || H.M > 128 || W.ca.max_singles > 96) return false;
+  size_t need_c = ((size_t)(W.n_a + 1) * nst + (size_t)H.M * H.M + (size_t)1024 * nst) * tsize;
   size_t need_r = ((size_t)W.n_b * nst + (size_t)H.M * H.M) * tsize;
   size_t lim = (size_t)c->lds_per_block;
   return need_c <= lim && need_r <= lim;

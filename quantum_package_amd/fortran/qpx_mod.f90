@@ -31,6 +31,11 @@ module qpx_mod
       character(kind=c_char) :: buf(*)
       integer(c_int), value :: n
     end function
+    integer(c_int) function qpx_set_verbosity(ctx, level) bind(C, name="qpx_set_verbosity")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: level
+    end function
     integer(c_int) function qpx_set_hamiltonian(ctx, mo_num, na, nb, h, eri, n8) bind(C, name="qpx_set_hamiltonian")
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: ctx

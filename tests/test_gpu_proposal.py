@@ -78,10 +78,11 @@ def test_learned_proposal():
 
 
 def test_cipsi_with_proposer():
-    """The fci_zmq loop with the proposer: full sweeps train it and measure
-    what it would have missed; the iterations in between enumerate the
-    proposed blocks only.  At the same N_det the variational energy stays
-    within 0.1 % of the correlation energy (E - E_HF) of the plain loop."""
+    """The fci_zmq loop with the proposer: full sweeps train it; the
+    iterations in between enumerate the proposed blocks only, and an audit
+    sweep measures the PT2 they missed (< 0.5 %).  At the same N_det the
+    variational energy stays within 0.1 % of the correlation energy
+    (E - E_HF) of the plain loop."""
     from quantum_package_amd.api import BlockMLP, Context
     H = synthetic_hamiltonian(24, 5, 5, seed=505)
     ctx = Context(0)
@@ -89,15 +90,15 @@ def test_cipsi_with_proposer():
     rp = cipsi(ctx, H, n_det_max=6000, s2_eig=False, log=_say)
     mlp = BlockMLP(n_orb=H.M, seed=7)
     rl = cipsi(ctx, H, n_det_max=6000, s2_eig=False, log=_say, proposer=Proposer(mlp, epochs=30),
-               full_sweep_every=2)
+               full_sweep_every=2, audit_proposal=True)
     mlp.close()
     ctx.close()
     prop = [h for h in rl.history if h.get("sweep") == "proposed"]
-    checks = [h["missed_pt2_fraction"] for h in rl.history if "missed_pt2_fraction" in h]
-    _say(f"  kept per proposed iteration {[round(h['kept'], 3) for h in prop]}; missed at full sweeps {checks}")
+    checks = [h["missed_pt2_fraction"] for h in prop]
+    _say(f"  kept per proposed iteration {[round(h['kept'], 3) for h in prop]}; missed {checks}")
     _say(f"  E plain {rp.energies[0]:.10f}  E proposer {rl.energies[0]:.10f}")
     assert prop and all(h["kept"] < 0.8 for h in prop)
-    assert checks and max(checks) < 1e-2
+    assert max(checks) < 5e-3
     assert rl.dets.shape[0] == rp.dets.shape[0]
     e_corr = rp.energies[0] - rp.history[0]["energies"][0]
     assert abs(rl.energies[0] - rp.energies[0]) < 1e-3 * abs(e_corr)
