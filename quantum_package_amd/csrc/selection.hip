@@ -382,6 +382,7 @@ struct SelArgs {
   int64_t pool_cap;
   double tau;
   int blk_base;   // global id of blocks[0]
+  int qcap;       // entries of the LDS selector queue
   // outputs per block
   double *blk_pt2;     // n_blocks * NS
   int *blk_status;     // 0 done, 1 skipped (owned by an earlier generator), 2 overflow
@@ -444,9 +445,23 @@ struct Ctx {
   }
 };
 
+// Couplings of one selector with four particles outside the block's mask
+// (get_d2, selection.irp.f:738-853): at most 6 cells, each met once,
+// returned as (cell, value) pairs in the reference's order.
+struct D2List {
+  int n;
+  uint32_t cell[6];   // i | j << 8 (1-based orbitals)
+  double hij[6];
+  __device__ inline void push(int i, int j, double v) {
+    cell[n] = (uint32_t)i | ((uint32_t)j << 8);
+    hij[n] = v;
+    ++n;
+  }
+};
+
 template <int NI>
 __device__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const int (*h)[2], const int (*p)[4], const int *nh,
-                       const int *np, int sp, const double *coefs) {
+                       const int *np, int sp, D2List &out) {
   // p[s][k] = k+1-th particle of spin s+1; h similarly
 #define P(n, s) p[(s) - 1][(n) - 1]
 #define H(n, s) h[(s) - 1][(n) - 1]
@@ -466,12 +481,11 @@ __device__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const int (*h)[2], const
       for (int i = 1; i <= 3; ++i) {
         int putj = P(i, ma);
         int ci = ma == 1 ? putj : puti, cj = ma == 1 ? puti : putj;
-        if (!owns(ci, cj)) continue;
         if (ban_get(BB, M, putj, puti)) continue;
         int i1 = (i == 1) ? 2 : 1, i2 = (i == 3) ? 2 : 3;
         int p1 = P(i1, ma), p2 = P(i2, ma), h1 = H(1, ma), h2 = H(2, ma);
         double hij = (C.I(p1, p2, h1, h2) - C.I(p2, p1, h1, h2)) * phase_bi<NI>(sel, ma, ma, h1, p1, h2, p2);
-        C.add(ci, cj, coefs, hij);
+        out.push(ci, cj, hij);
       }
     } else {
       int h1 = H(1, 1), h2 = H(1, 2);
@@ -479,11 +493,10 @@ __device__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const int (*h)[2], const
         int putj = P(j, 2), p2 = P(3 - j, 2);
         for (int i = 1; i <= 2; ++i) {
           int puti = P(i, 1);
-          if (!owns(puti, putj)) continue;
           if (ban_get(BB, M, puti, putj)) continue;
           int p1 = P(3 - i, 1);
           double hij = C.I(p1, p2, h1, h2) * phase_bi<NI>(sel, 1, 2, h1, p1, h2, p2);
-          C.add(puti, putj, coefs, hij);
+          out.push(puti, putj, hij);
         }
       }
     }
@@ -494,14 +507,13 @@ __device__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const int (*h)[2], const
         int puti = P(i, ma);
         for (int j = i + 1; j <= 4; ++j) {
           int putj = P(j, ma);
-          if (!owns(puti, putj)) continue;
           if (ban_get(B1, M, puti, putj)) continue;
           int c[2], n = 0;
           for (int k = 1; k <= 4; ++k)
             if (k != i && k != j) c[n++] = k;
           int p1 = P(c[0], ma), p2 = P(c[1], ma);
           double hij = (C.I(p1, p2, h1, h2) - C.I(p2, p1, h1, h2)) * phase_bi<NI>(sel, ma, ma, h1, p1, h2, p2);
-          C.add(puti, putj, coefs, hij);
+          out.push(puti, putj, hij);
         }
       }
     } else if (tip == 3) {
@@ -510,18 +522,17 @@ __device__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const int (*h)[2], const
         int i1 = (i == 1) ? 2 : 1, i2 = (i == 3) ? 2 : 3;
         int puti = P(i1, ma), putj = P(i2, ma);
         int ci = min(puti, putj), cj = max(puti, putj);
-        if (!owns(ci, cj)) continue;
         if (ban_get(B1, M, puti, putj)) continue;
         int p2 = P(i, ma);
         double hij = C.I(p1, p2, h1, h2) * phase_bi<NI>(sel, mi, ma, h1, p1, h2, p2);
-        C.add(ci, cj, coefs, hij);
+        out.push(ci, cj, hij);
       }
     } else {
       int puti = P(1, sp), putj = P(2, sp);
-      if (owns(puti, putj) && !ban_get(B1, M, puti, putj)) {
+      if (!ban_get(B1, M, puti, putj)) {
         int p1 = P(1, mi), p2 = P(2, mi), h1 = H(1, mi), h2 = H(2, mi);
         double hij = (C.I(p1, p2, h1, h2) - C.I(p2, p1, h1, h2)) * phase_bi<NI>(sel, mi, mi, h1, p1, h2, p2);
-        C.add(puti, putj, coefs, hij);
+        out.push(puti, putj, hij);
       }
     }
   }
@@ -782,6 +793,18 @@ __device__ inline double diag_fock(const Tabs &T, const double *f, int sp, int h
 #undef JJ
 }
 
+// One selector in the LDS queue of a block: its index, kind (0: four
+// particles outside the mask, couplings precomputed; 1: three; 2: two or
+// fewer) with the number of precomputed couplings, the couplings' cells
+// (i | j << 8) and values, and its coefficients.
+struct SelQ {
+  int idx;
+  int kind_n;
+  uint16_t cell[6];
+  double hij[6];
+  double coef[MAXNS];
+};
+
 template <int NI>
 __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -794,7 +817,9 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
   L.ban2 = L.ban1 + nbits;
   L.borb = L.ban2 + nbits;
   L.flag = reinterpret_cast<int *>(L.borb + 16);
-  double *red = reinterpret_cast<double *>(L.flag + 4);  // SB * MAXNS doubles (8-aligned below)
+  int *wsum = L.flag + 4;  // per-wavefront counts of the queue compaction
+  // reduction scratch (SB * MAXNS doubles), shared with the selector queue
+  double *red = reinterpret_cast<double *>(wsum + 8);
   red = reinterpret_cast<double *>((reinterpret_cast<uintptr_t>(red) + 7) & ~uintptr_t(7));
 
   const BlockDesc B = A.blocks[blockIdx.x];
@@ -980,24 +1005,98 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
   // diagonal by past_d2 (never happens: past_d2 sets both orientations)
 
   // ---- splash_pq: couplings in selector order ----
+  // Phase A: SB near-list entries at a time are filtered in parallel and the
+  // survivors appended, in near-list order, to an LDS queue; the filtering
+  // thread also computes the couplings of a four-particle selector
+  // (get_d2).  Phase B (flush): the queue in order; a four-particle entry
+  // adds its precomputed values in the cells' owner threads, the others run
+  // get_d1 / get_d0 with every thread.  Each cell thus receives its terms in
+  // selector order, as in the reference.
   Ctx<NI> C(T, A, L, mask);
-  for (int x = 0; x < n_near; ++x) {
-    const int idx = near[x];
-    if (idx < B.gen || idx > A.n_sel) continue;
-    const uint64_t *d = A.psi + (int64_t)(idx - 1) * 2 * NI;
-    int nt = 0;
+  SelQ *Q = reinterpret_cast<SelQ *>(red);
+  auto flush = [&](int nq) {
+    for (int k = 0; k < nq; ++k) {
+      const SelQ &q = Q[k];
+      const int kind = q.kind_n & 0xf;
+      if (kind == 0) {
+        const int nc = q.kind_n >> 4;
+        for (int c = 0; c < nc; ++c) {
+          const int ci = q.cell[c] & 0xff, cj = q.cell[c] >> 8;
+          if (owns(ci, cj)) C.add(ci, cj, q.coef, q.hij[c]);
+        }
+      } else {
+        const uint64_t *d = A.psi + (int64_t)(q.idx - 1) * 2 * NI;
+        int p[2][4], np[2], h[2][2], nh[2];
+        sel_lists<NI>(mask, d, p, np, h, nh);
+        if (kind == 1) get_d1<NI>(C, d, h, p, nh, np, sp, q.coef, A.na, A.nb);
+        else get_d0<NI>(C, d, p, sp, q.coef, A.na, A.nb);
+      }
+    }
+  };
+  const int lane = tid & 63, wv = tid >> 6;
+  int qn = 0;
+  for (int x0 = 0; x0 < n_near; x0 += SB) {
+    const int x = x0 + tid;
+    bool pass = false;
+    int idx = 0, nt = 0;
+    if (x < n_near) {
+      idx = near[x];
+      if (idx >= B.gen && idx <= A.n_sel) {
+        const uint64_t *d = A.psi + (int64_t)(idx - 1) * 2 * NI;
 #pragma unroll
-    for (int w = 0; w < 2 * NI; ++w) nt += __popcll(~mask[w] & d[w]);
-    if (nt > 4) continue;
-    int p[2][4], np[2], h[2][2], nh[2];
-    sel_lists<NI>(mask, d, p, np, h, nh);
+        for (int w = 0; w < 2 * NI; ++w) nt += __popcll(~mask[w] & d[w]);
+        pass = nt <= 4;
+      }
+    }
+    D2List dl;
+    dl.n = 0;
     double coefs[MAXNS];
-    for (int st = 0; st < NS; ++st) coefs[st] = A.coef[(idx - 1) + A.n_det * st];
-    if (tid == 0) C.gath += NS;
-    if (nt == 4) get_d2<NI>(C, d, h, p, nh, np, sp, coefs);
-    else if (nt == 3) get_d1<NI>(C, d, h, p, nh, np, sp, coefs, A.na, A.nb);
-    else get_d0<NI>(C, d, p, sp, coefs, A.na, A.nb);
+    if (pass) {
+      const uint64_t *d = A.psi + (int64_t)(idx - 1) * 2 * NI;
+      for (int st = 0; st < NS; ++st) coefs[st] = A.coef[(idx - 1) + A.n_det * st];
+      C.gath += NS;
+      if (nt == 4) {
+        int p[2][4], np[2], h[2][2], nh[2];
+        sel_lists<NI>(mask, d, p, np, h, nh);
+        get_d2<NI>(C, d, h, p, nh, np, sp, dl);
+      }
+    }
+    // order-preserving compaction: position of this survivor in the chunk
+    const unsigned long long bal = __ballot(pass);
+    const int before = __popcll(bal & ((1ULL << lane) - 1ULL));
+    if (lane == 0) wsum[wv] = __popcll(bal);
+    __syncthreads();
+    int off = 0, cnt = 0;
+    for (int w = 0; w < SB / 64; ++w) {
+      if (w < wv) off += wsum[w];
+      cnt += wsum[w];
+    }
+    const int pos = off + before;
+    int done = 0;
+    while (true) {
+      const int space = A.qcap - qn;
+      if (pass && pos >= done && pos < done + space) {
+        SelQ &q = Q[qn + pos - done];
+        q.idx = idx;
+        q.kind_n = (nt == 4 ? 0 : (nt == 3 ? 1 : 2)) | (dl.n << 4);
+        for (int c = 0; c < dl.n; ++c) {
+          q.cell[c] = (uint16_t)dl.cell[c];
+          q.hij[c] = dl.hij[c];
+        }
+        for (int st = 0; st < NS; ++st) q.coef[st] = coefs[st];
+      }
+      const int adv = min(cnt - done, space);
+      qn += adv;
+      done += adv;
+      if (done == cnt) break;
+      __syncthreads();
+      flush(qn);
+      __syncthreads();
+      qn = 0;
+    }
+    __syncthreads();
   }
+  flush(qn);
   __syncthreads();
 
   // ---- fill_buffer_double ----
@@ -1125,10 +1224,16 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
   const int M = H.M;
   // LDS budget
   const int nbits = (M * M + 31) / 32;
-  size_t lds = sizeof(double) * ((size_t)NS * M * M + 2 * (M + 1)) + sizeof(uint32_t) * (2 * nbits + 16) +
-               sizeof(int) * 4 + 8 + sizeof(double) * SB * std::max(NS, 1);
-  QPX_REQUIRE(lds <= (size_t)c->lds_per_block,
+  // mat | fock | ban1, ban2, bannedOrb | flags, wavefront counts | reduction
+  // scratch shared with the selector queue
+  const size_t lds_base = sizeof(double) * ((size_t)NS * M * M + 2 * (M + 1)) + sizeof(uint32_t) * (2 * nbits + 16) +
+                          sizeof(int) * 12 + 8;
+  const size_t red_bytes = sizeof(double) * SB * MAXNS;
+  QPX_REQUIRE(lds_base + red_bytes <= (size_t)c->lds_per_block,
               "selection: N_states * mo_num^2 coupling matrix does not fit in LDS of one workgroup");
+  const int qcap = (int)std::min<size_t>(1024, ((size_t)c->lds_per_block - lds_base) / sizeof(SelQ));
+  QPX_REQUIRE(qcap >= 16, "selection: no LDS left for the selector queue");
+  const size_t lds = lds_base + std::max(red_bytes, (size_t)qcap * sizeof(SelQ));
   auto kfun = select_block_kernel<NI>;
   QPX_HIP(hipFuncSetAttribute((const void *)kfun, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 
@@ -1309,6 +1414,7 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
       // the pool records run-local block ids; they are translated to global
       // ids right after the launch
       SA.blk_base = 0;
+      SA.qcap = qcap;
       // remember the mapping run index -> global block id for drops
       std::vector<int64_t> run_gid(nb);
       for (int i = 0; i < nb; ++i) run_gid[i] = bl_gid[pending[i]];
