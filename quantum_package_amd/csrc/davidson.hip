@@ -74,10 +74,13 @@ __global__ void from_t_kernel(const T *__restrict__ xt, int64_t n, int nb, doubl
 
 // partial[blk][i*nb+j] = sum over the block's rows of A(k,i) B(k,j)
 // (na, nb <= 32), rows staged through LDS in tiles of 64
+// B2 (optional): nb more columns, so that one pass over A serves A^T [B B2]
 __global__ __launch_bounds__(RB) void gram_partial_kernel(const double *__restrict__ A, int64_t lda, int na,
-                                                          const double *__restrict__ B, int64_t ldb, int nb,
+                                                          const double *__restrict__ B,
+                                                          const double *__restrict__ B2, int64_t ldb, int nb1,
                                                           int64_t n, double *__restrict__ partial) {
   __shared__ double ta[64 * 33], tb[64 * 33];
+  const int nb = B2 ? 2 * nb1 : nb1;
   const int nout = na * nb;
   double acc[4] = {0, 0, 0, 0};
   int64_t rows_per = (n + gridDim.x - 1) / gridDim.x;
@@ -91,7 +94,8 @@ __global__ __launch_bounds__(RB) void gram_partial_kernel(const double *__restri
     }
     for (int x = threadIdx.x; x < 64 * nb; x += RB) {
       int j = x / 64, r = x - j * 64;
-      tb[r * 33 + j] = (r < m) ? B[t0 + r + ldb * j] : 0.0;
+      const double *src = (j < nb1) ? B + ldb * j : B2 + ldb * (j - nb1);
+      tb[r * 33 + j] = (r < m) ? src[t0 + r] : 0.0;
     }
     __syncthreads();
 #pragma unroll
@@ -228,6 +232,30 @@ __global__ void gemm_y_kernel(const double *__restrict__ A, int64_t lda, int nco
   }
 }
 
+// X(:, 0:nb) <- X(:, 0:nb) R (R upper triangular nb x nb, column-major),
+// row by row in place
+__global__ void tri_mul_kernel(double *__restrict__ X, int64_t ld, int nb, const double *__restrict__ R, int64_t n) {
+  __shared__ double rs[8 * 8];
+  for (int x = threadIdx.x; x < nb * nb; x += blockDim.x) rs[x] = R[x];
+  __syncthreads();
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    double a[8], o[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = (i < nb) ? X[k + ld * i] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      double t = 0.0;
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if (i <= j && j < nb) t += a[i] * rs[i + nb * j];
+      o[j] = t;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (j < nb) X[k + ld * j] = o[j];
+  }
+}
+
 // ---- small dense helpers on the host ----
 void jacobi_eigh(int n, const std::vector<double> &Ain, int lda, std::vector<double> &w, std::vector<double> &V) {
   std::vector<double> A((size_t)n * n);
@@ -298,10 +326,12 @@ struct Dav {
     grid_rows = (int)std::min<int64_t>((n + 255) / 256, 8192);
   }
 
-  // C (na x nb, column-major, host) = A^T B
-  void gram(const double *A, int64_t lda, int na, const double *B, int64_t ldb, int nb, std::vector<double> &C) {
+  // C (na x nb, column-major, host) = A^T B; with B2, C is na x 2nb = A^T [B B2]
+  void gram(const double *A, int64_t lda, int na, const double *B, int64_t ldb, int nb, std::vector<double> &C,
+            const double *B2 = nullptr) {
     int nblk = (int)std::min<int64_t>(RBLOCKS, std::max<int64_t>(1, n / 256));
-    hipLaunchKernelGGL(gram_partial_kernel, dim3(nblk), dim3(RB), 0, st, A, lda, na, B, ldb, nb, n, part.p);
+    hipLaunchKernelGGL(gram_partial_kernel, dim3(nblk), dim3(RB), 0, st, A, lda, na, B, B2, ldb, nb, n, part.p);
+    if (B2) nb *= 2;
     int nout = na * nb;
     hipLaunchKernelGGL(reduce_partial_kernel, dim3((nout + 255) / 256), dim3(256), 0, st, part.p, nblk, nout,
                        small.p);
@@ -321,6 +351,42 @@ struct Dav {
     return C[0];
   }
 
+  // X (n x nb) <- X R^-1 with X^T X = R^T R; false (X untouched) when the
+  // Cholesky factorisation breaks down
+  bool chol_qr(double *X, int nb) {
+    std::vector<double> G;
+    gram(X, n, nb, X, n, nb, G);
+    std::vector<double> R((size_t)nb * nb, 0.0);
+    for (int j = 0; j < nb; ++j) {
+      for (int i = 0; i <= j; ++i) {
+        double t = G[i + (size_t)nb * j];
+        for (int k = 0; k < i; ++k) t -= R[k + (size_t)nb * i] * R[k + (size_t)nb * j];
+        if (i == j) {
+          if (!(t > 1e-20 * std::max(1.0, G[j + (size_t)nb * j]))) return false;
+          R[j + (size_t)nb * j] = std::sqrt(t);
+        } else {
+          R[i + (size_t)nb * j] = t / R[i + (size_t)nb * i];
+        }
+      }
+    }
+    // R^-1 (upper triangular)
+    std::vector<double> Ri((size_t)nb * nb, 0.0);
+    for (int j = 0; j < nb; ++j) {
+      Ri[j + (size_t)nb * j] = 1.0 / R[j + (size_t)nb * j];
+      for (int i = j - 1; i >= 0; --i) {
+        double t = 0.0;
+        for (int k = i + 1; k <= j; ++k) t += R[i + (size_t)nb * k] * Ri[k + (size_t)nb * j];
+        Ri[i + (size_t)nb * j] = -t / R[i + (size_t)nb * i];
+      }
+    }
+    for (double v : Ri)
+      if (!std::isfinite(v)) return false;
+    QPX_HIP(hipMemcpyAsync(small.p, Ri.data(), sizeof(double) * Ri.size(), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(tri_mul_kernel, dim3(grid_rows), dim3(256), 0, st, X, n, nb, small.p, n);
+    QPX_HIP(hipGetLastError());
+    return true;
+  }
+
   // orthonormalise columns [c0, c0+nb) of U against [0, c0) and each other
   void orthonormalize(double *U, int c0, int nb) {
     std::vector<double> C;
@@ -333,6 +399,10 @@ struct Dav {
         QPX_HIP(hipGetLastError());
       }
     }
+    // within the block: Cholesky QR, twice (same orthonormal basis as
+    // Gram-Schmidt in column order); Gram-Schmidt when the block is too
+    // close to rank deficient for the Cholesky factor
+    if (chol_qr(U + n * c0, nb) && chol_qr(U + n * c0, nb)) return;
     for (int j = 0; j < nb; ++j) {
       double *x = U + n * (c0 + j);
       for (int pass = 0; pass < 2; ++pass) {
@@ -415,7 +485,8 @@ extern "C" int qpx_davidson(qpx_ctx *c, int N_st, int N_st_diag, double threshol
     lamdev.alloc(8);
     s2dev.alloc(8);
 
-    std::vector<double> h, s_, y((size_t)nc * nc, 0.0), lambda(nc, 0.0), s2(nc, 0.0), resid(N_st_diag, 0.0);
+    std::vector<double> hm((size_t)nc * nc, 0.0), sm((size_t)nc * nc, 0.0), y((size_t)nc * nc, 0.0),
+        lambda(nc, 0.0), s2(nc, 0.0), resid(N_st_diag, 0.0);
     bool use_fp32 = mixed != 0;
     int n64 = 0, n32 = 0;
     bool converged = false;
@@ -449,10 +520,27 @@ extern "C" int qpx_davidson(qpx_ctx *c, int N_st, int N_st_diag, double threshol
           }
           QPX_HIP(hipGetLastError());
         }
-        D.gram(U.p, n, shift2, W.p, n, shift2, h);
-        D.gram(U.p, n, shift2, S.p, n, shift2, s_);
+        // h = U^T W and s = U^T S: the columns before the new block are
+        // unchanged since the last iteration, so only U^T [W S](:, new
+        // block) is formed; h and s are kept symmetric
+        {
+          std::vector<double> G;
+          D.gram(U.p, n, shift2, W.p + n * shift, n, N_st_diag, G, S.p + n * shift);
+          for (int j = 0; j < N_st_diag; ++j) {
+            const int cj = shift + j;
+            for (int i = 0; i < shift2; ++i) {
+              const double hv = G[i + (size_t)shift2 * j], sv = G[i + (size_t)shift2 * (N_st_diag + j)];
+              hm[i + (size_t)nc * cj] = hv;
+              sm[i + (size_t)nc * cj] = sv;
+              if (i < shift) {
+                hm[cj + (size_t)nc * i] = hv;
+                sm[cj + (size_t)nc * i] = sv;
+              }
+            }
+          }
+        }
         std::vector<double> w, yy;
-        jacobi_eigh(shift2, h, shift2, w, yy);
+        jacobi_eigh(shift2, hm, nc, w, yy);
         for (int j = 0; j < shift2; ++j) {
           lambda[j] = w[j];
           for (int i = 0; i < shift2; ++i) y[i + (size_t)nc * j] = yy[i + (size_t)shift2 * j];
@@ -462,7 +550,7 @@ extern "C" int qpx_davidson(qpx_ctx *c, int N_st, int N_st_diag, double threshol
           double acc = 0.0;
           for (int i = 0; i < shift2; ++i) {
             double t = 0.0;
-            for (int l = 0; l < shift2; ++l) t += s_[i + (size_t)shift2 * l] * y[l + (size_t)nc * k];
+            for (int l = 0; l < shift2; ++l) t += sm[i + (size_t)nc * l] * y[l + (size_t)nc * k];
             acc += y[i + (size_t)nc * k] * t;
           }
           s2[k] = acc + sz2;

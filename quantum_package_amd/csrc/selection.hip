@@ -13,10 +13,15 @@
 //     preinteresting / prefullinteresting lists);
 //   * one workgroup owns one (generator, h1, h2) block -- the reference's
 //     maskInd unit.  The block's coupling matrix mat(N_states, M, M) lives
-//     in LDS; cell (i,j) is owned by thread (i+j) mod 256, so every line of
-//     cells touched by get_d1 maps to distinct threads and each cell
-//     receives its terms in selector order, the same order as the
-//     reference (bit-identical FP64 sums);
+//     in LDS; with the rows split into two bands, cell (i,j) is owned by
+//     thread 256 * band(i) + (i+j-2) mod 256, so every line of cells
+//     touched by get_d1 maps to distinct threads and each cell receives its
+//     terms in selector order, the same order as the reference
+//     (bit-identical FP64 sums);
+//   * the block's selectors are filtered in parallel into an LDS queue in
+//     near-list order; the queue is then replayed in order, the couplings
+//     of four-particle selectors having been computed by the filtering
+//     thread;
 //   * bans (spot_isinwf, past_d1/d2, symmetrisation) are order-independent
 //     sets and are built in parallel with LDS atomics before any coupling is
 //     accumulated;
@@ -39,7 +44,9 @@ using namespace qpx;
 
 namespace {
 
-constexpr int SB = 256;  // threads per block workgroup
+constexpr int SB = 512;             // threads per block workgroup
+constexpr int OW = 256;             // owners per band of rows
+constexpr int NBAND = SB / OW;      // bands of rows of the coupling matrix
 constexpr int MAXNS = 4;
 
 // ---------------------------------------------------------------- helpers
@@ -423,7 +430,33 @@ __device__ inline void borb_set(uint32_t *b, int o, int s) {
 }
 
 // owner of mat cell (i,j) (1-based): thread (i+j-2) mod SB
-__device__ inline bool owns(int i, int j) { return ((i + j - 2) & (SB - 1)) == (int)threadIdx.x; }
+// Ownership of the coupling-matrix cells: the M rows are split into NBAND
+// bands; cell (i, j) of band b belongs to thread OW * b + ((i + j - 2) mod
+// OW).  Every row and every column then has distinct owners (M <= OW), so a
+// thread meets at most one cell of any line.
+static_assert(NBAND == 2, "band_of assumes two bands");
+__device__ inline int band_of(int i, int M) { return (2 * (i - 1) >= M) ? 1 : 0; }   // floor(2 (i-1) / M)
+__device__ inline bool owns(int i, int j, int M) {
+  return ((i + j - 2) & (OW - 1)) + OW * band_of(i, M) == (int)threadIdx.x;
+}
+// the unique j (1..M) such that thread owns (i, j); 0 if none
+__device__ inline int owned_j(int i, int M) {
+  if (band_of(i, M) != (int)threadIdx.x / OW) return 0;
+  const int j = (((int)threadIdx.x & (OW - 1)) - (i - 1)) & (OW - 1);
+  return (j < M) ? j + 1 : 0;
+}
+// the unique i (1..M) such that thread owns (i, j); 0 if none
+__device__ inline int owned_i(int j, int M) {
+  const int i0 = (((int)threadIdx.x & (OW - 1)) - (j - 1)) & (OW - 1);
+  if (i0 >= M) return 0;
+  return (band_of(i0 + 1, M) == (int)threadIdx.x / OW) ? i0 + 1 : 0;
+}
+// rows [lo, hi] of this thread's band
+__device__ inline void band_rows(int M, int &lo, int &hi) {
+  const int b = (int)threadIdx.x / OW;
+  lo = (b * M + 1) / 2 + 1;        // smallest i with band_of(i) == b
+  hi = ((b + 1) * M + 1) / 2;      // largest such i
+}
 
 template <int NI>
 struct Ctx {
@@ -447,25 +480,30 @@ struct Ctx {
 
 // Couplings of one selector with four particles outside the block's mask
 // (get_d2, selection.irp.f:738-853): at most 6 cells, each met once,
-// returned as (cell, value) pairs in the reference's order.
+// written as (cell, value) pairs in the reference's order straight into the
+// selector's LDS queue entry.
 struct D2List {
+  uint32_t *cellw;   // 3 words, two cells i | j << 8 (1-based) per word
+  double *hij;
   int n;
-  uint32_t cell[6];   // i | j << 8 (1-based orbitals)
-  double hij[6];
   __device__ inline void push(int i, int j, double v) {
-    cell[n] = (uint32_t)i | ((uint32_t)j << 8);
+    const uint32_t c = (uint32_t)i | ((uint32_t)j << 8);
+    if (n & 1) cellw[n >> 1] |= c << 16;
+    else cellw[n >> 1] = c;
     hij[n] = v;
     ++n;
   }
 };
 
 template <int NI>
-__device__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const int (*h)[2], const int (*p)[4], const int *nh,
-                       const int *np, int sp, D2List &out) {
-  // p[s][k] = k+1-th particle of spin s+1; h similarly
+__device__ __forceinline__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const int (*h)[2], const int (*p)[4],
+                                       const int *nh, const int *np, int sp, D2List &out) {
+  // p[s][k] = k+1-th particle of spin s+1; h similarly.  The (at most 6)
+  // cells are collected first, their integrals loaded together, then the
+  // values pushed in the reference's order: (V[ia] - V[ib]) * sign, or
+  // V[ia] * sign when ib < 0.
 #define P(n, s) p[(s) - 1][(n) - 1]
 #define H(n, s) h[(s) - 1][(n) - 1]
-  int bant = 1;
   int tip = np[0] * np[1];
   int ma = sp;
   if (np[0] > np[1]) ma = 1;
@@ -473,80 +511,133 @@ __device__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const int (*h)[2], const
   int mi = (ma % 2) + 1;
   const uint32_t *B1 = C.L.ban1;
   const int M = C.M;
+  bool sv[6];
+  int sci[6], scj[6];
+  int64_t sia[6], sib[6];
+  double sg[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) sv[k] = false;
+  auto idx4 = [&](int i, int j, int k, int l) -> int64_t {   // <ij|kl>, 1-based
+    return pair_index(i - 1, k - 1) * (int64_t)C.T.npair + pair_index(j - 1, l - 1);
+  };
   if (sp == 3) {
-    if (ma == 2) bant = 2;
-    const uint32_t *BB = bant == 1 ? C.L.ban1 : C.L.ban2;
+    const uint32_t *BB = (ma == 2) ? C.L.ban2 : C.L.ban1;
     if (tip == 3) {
-      int puti = P(1, mi);
+      const int puti = P(1, mi);
+#pragma unroll
       for (int i = 1; i <= 3; ++i) {
-        int putj = P(i, ma);
-        int ci = ma == 1 ? putj : puti, cj = ma == 1 ? puti : putj;
+        const int putj = P(i, ma);
         if (ban_get(BB, M, putj, puti)) continue;
-        int i1 = (i == 1) ? 2 : 1, i2 = (i == 3) ? 2 : 3;
-        int p1 = P(i1, ma), p2 = P(i2, ma), h1 = H(1, ma), h2 = H(2, ma);
-        double hij = (C.I(p1, p2, h1, h2) - C.I(p2, p1, h1, h2)) * phase_bi<NI>(sel, ma, ma, h1, p1, h2, p2);
-        out.push(ci, cj, hij);
+        const int i1 = (i == 1) ? 2 : 1, i2 = (i == 3) ? 2 : 3;
+        const int p1 = P(i1, ma), p2 = P(i2, ma), h1 = H(1, ma), h2 = H(2, ma);
+        const int k = i - 1;
+        sv[k] = true;
+        sci[k] = ma == 1 ? putj : puti;
+        scj[k] = ma == 1 ? puti : putj;
+        sia[k] = idx4(p1, p2, h1, h2);
+        sib[k] = idx4(p2, p1, h1, h2);
+        sg[k] = phase_bi<NI>(sel, ma, ma, h1, p1, h2, p2);
       }
     } else {
-      int h1 = H(1, 1), h2 = H(1, 2);
+      const int h1 = H(1, 1), h2 = H(1, 2);
+#pragma unroll
       for (int j = 1; j <= 2; ++j) {
-        int putj = P(j, 2), p2 = P(3 - j, 2);
+        const int putj = P(j, 2), p2 = P(3 - j, 2);
+#pragma unroll
         for (int i = 1; i <= 2; ++i) {
-          int puti = P(i, 1);
+          const int puti = P(i, 1);
           if (ban_get(BB, M, puti, putj)) continue;
-          int p1 = P(3 - i, 1);
-          double hij = C.I(p1, p2, h1, h2) * phase_bi<NI>(sel, 1, 2, h1, p1, h2, p2);
-          out.push(puti, putj, hij);
+          const int p1 = P(3 - i, 1);
+          const int k = 2 * (j - 1) + (i - 1);
+          sv[k] = true;
+          sci[k] = puti;
+          scj[k] = putj;
+          sia[k] = idx4(p1, p2, h1, h2);
+          sib[k] = -1;
+          sg[k] = phase_bi<NI>(sel, 1, 2, h1, p1, h2, p2);
         }
       }
     }
   } else {
     if (tip == 0) {
-      int h1 = H(1, ma), h2 = H(2, ma);
+      const int h1 = H(1, ma), h2 = H(2, ma);
+      int k = 0;
+#pragma unroll
       for (int i = 1; i <= 3; ++i) {
-        int puti = P(i, ma);
+        const int puti = P(i, ma);
+#pragma unroll
         for (int j = i + 1; j <= 4; ++j) {
-          int putj = P(j, ma);
+          const int putj = P(j, ma);
+          const int kk = k++;
           if (ban_get(B1, M, puti, putj)) continue;
           int c[2], n = 0;
-          for (int k = 1; k <= 4; ++k)
-            if (k != i && k != j) c[n++] = k;
-          int p1 = P(c[0], ma), p2 = P(c[1], ma);
-          double hij = (C.I(p1, p2, h1, h2) - C.I(p2, p1, h1, h2)) * phase_bi<NI>(sel, ma, ma, h1, p1, h2, p2);
-          out.push(puti, putj, hij);
+#pragma unroll
+          for (int q = 1; q <= 4; ++q)
+            if (q != i && q != j) c[n++] = q;
+          const int p1 = P(c[0], ma), p2 = P(c[1], ma);
+          sv[kk] = true;
+          sci[kk] = puti;
+          scj[kk] = putj;
+          sia[kk] = idx4(p1, p2, h1, h2);
+          sib[kk] = idx4(p2, p1, h1, h2);
+          sg[kk] = phase_bi<NI>(sel, ma, ma, h1, p1, h2, p2);
         }
       }
     } else if (tip == 3) {
-      int h1 = H(1, mi), h2 = H(1, ma), p1 = P(1, mi);
+      const int h1 = H(1, mi), h2 = H(1, ma), p1 = P(1, mi);
+#pragma unroll
       for (int i = 1; i <= 3; ++i) {
-        int i1 = (i == 1) ? 2 : 1, i2 = (i == 3) ? 2 : 3;
-        int puti = P(i1, ma), putj = P(i2, ma);
-        int ci = min(puti, putj), cj = max(puti, putj);
+        const int i1 = (i == 1) ? 2 : 1, i2 = (i == 3) ? 2 : 3;
+        const int puti = P(i1, ma), putj = P(i2, ma);
         if (ban_get(B1, M, puti, putj)) continue;
-        int p2 = P(i, ma);
-        double hij = C.I(p1, p2, h1, h2) * phase_bi<NI>(sel, mi, ma, h1, p1, h2, p2);
-        out.push(ci, cj, hij);
+        const int p2 = P(i, ma);
+        const int k = i - 1;
+        sv[k] = true;
+        sci[k] = min(puti, putj);
+        scj[k] = max(puti, putj);
+        sia[k] = idx4(p1, p2, h1, h2);
+        sib[k] = -1;
+        sg[k] = phase_bi<NI>(sel, mi, ma, h1, p1, h2, p2);
       }
     } else {
-      int puti = P(1, sp), putj = P(2, sp);
+      const int puti = P(1, sp), putj = P(2, sp);
       if (!ban_get(B1, M, puti, putj)) {
-        int p1 = P(1, mi), p2 = P(2, mi), h1 = H(1, mi), h2 = H(2, mi);
-        double hij = (C.I(p1, p2, h1, h2) - C.I(p2, p1, h1, h2)) * phase_bi<NI>(sel, mi, mi, h1, p1, h2, p2);
-        out.push(puti, putj, hij);
+        const int p1 = P(1, mi), p2 = P(2, mi), h1 = H(1, mi), h2 = H(2, mi);
+        sv[0] = true;
+        sci[0] = puti;
+        scj[0] = putj;
+        sia[0] = idx4(p1, p2, h1, h2);
+        sib[0] = idx4(p2, p1, h1, h2);
+        sg[0] = phase_bi<NI>(sel, mi, mi, h1, p1, h2, p2);
       }
     }
   }
+  double va[6], vb[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    va[k] = sv[k] ? C.T.V[sia[k]] : 0.0;
+    vb[k] = (sv[k] && sib[k] >= 0) ? C.T.V[sib[k]] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    if (!sv[k]) continue;
+    double hij;
+    if (sib[k] >= 0) {
+      C.gath += 2;
+      hij = (va[k] - vb[k]) * sg[k];
+    } else {
+      C.gath += 1;
+      hij = va[k] * sg[k];
+    }
+    out.push(sci[k], scj[k], hij);
+  }
 }
 
-// the unique j (1..M) such that thread owns (i, j); 0 if none
-__device__ inline int owned_j(int i, int M) {
-  int j = ((int)threadIdx.x - (i - 1)) & (SB - 1);
-  return (j < M) ? j + 1 : 0;
-}
 
 template <int NI>
-__device__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const int (*h)[2], const int (*p)[4], const int *nh,
-                       const int *np, int sp, const double *coefs, int na_e, int nb_e) {
+__device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const int (*h)[2], const int (*p)[4],
+                                       const int *nh, const int *np, int sp, const double *coefs, int na_e,
+                                       int nb_e) {
   const int M = C.M;
   const uint32_t *BO = C.L.borb;
   const uint32_t *B1 = C.L.ban1;
@@ -560,99 +651,139 @@ __device__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const int (*h)[2], const
   int ma = 1;
   if (np[1] >= 2) ma = 2;
   int mi = (ma == 1) ? 2 : 1;
-  int bant = 1;
+  // The cells this thread owns on the selector's lines (at most 6) are
+  // collected first, their integrals loaded together, then added: value =
+  // (V[ia] - V[ib]) * sign, or V[ia] * sign when ib < 0.
+  bool sv[6];
+  int sci[6], scj[6];
+  int64_t sia[6], sib[6];
+  double sg[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) sv[k] = false;
+  auto idx4 = [&](int i, int j, int k, int l) -> int64_t {   // <ij|kl>, 1-based
+    return pair_index(i - 1, k - 1) * (int64_t)C.T.npair + pair_index(j - 1, l - 1);
+  };
+  auto row_of = [&](int j) -> int { return owned_i(j, M); };   // i with owns(i, j), 0 if none
   if (sp == 3) {
-    if (ma == 2) bant = 2;
-    const uint32_t *BB = bant == 1 ? C.L.ban1 : C.L.ban2;
-    int puti = P(1, mi), hfix = H(1, ma), p1 = P(1, ma), p2 = P(2, ma);
+    const uint32_t *BB = (ma == 2) ? C.L.ban2 : C.L.ban1;
+    const int puti = P(1, mi), hfix = H(1, ma), p1 = P(1, ma), p2 = P(2, ma);
     if (!borb_get(BO, puti, mi)) {
-      // cells (putj, puti) if ma == 1 else (puti, putj): owned putj is unique
-      int putj = (ma == 1) ? 0 : owned_j(puti, M);
-      if (ma == 1) {
-        int j = ((int)threadIdx.x - (puti - 1)) & (SB - 1);  // i-1 for cell (i, puti)
-        putj = (j < M) ? j + 1 : 0;
-      }
+      const int putj = (ma == 1) ? row_of(puti) : owned_j(puti, M);
       if (putj && putj != hfix && !lbanned(putj, ma) && !ban_get(BB, M, putj, puti)) {
-        double hij;
-        if (putj < hfix)
-          hij = (C.I(p1, p2, putj, hfix) - C.I(p2, p1, putj, hfix)) * phase_bi<NI>(sel, ma, ma, putj, p1, hfix, p2);
-        else
-          hij = (C.I(p1, p2, hfix, putj) - C.I(p2, p1, hfix, putj)) * phase_bi<NI>(sel, ma, ma, hfix, p1, putj, p2);
-        if (ma == 1) C.add(putj, puti, coefs, hij);
-        else C.add(puti, putj, coefs, hij);
+        sv[0] = true;
+        sci[0] = (ma == 1) ? putj : puti;
+        scj[0] = (ma == 1) ? puti : putj;
+        if (putj < hfix) {
+          sia[0] = idx4(p1, p2, putj, hfix);
+          sib[0] = idx4(p2, p1, putj, hfix);
+          sg[0] = phase_bi<NI>(sel, ma, ma, putj, p1, hfix, p2);
+        } else {
+          sia[0] = idx4(p1, p2, hfix, putj);
+          sib[0] = idx4(p2, p1, hfix, putj);
+          sg[0] = phase_bi<NI>(sel, ma, ma, hfix, p1, putj, p2);
+        }
       }
     }
-    int pfix = P(1, mi);
-    // mi part: lines with fixed ma-orbital p1 and p2, varying mi-orbital
+    const int pfix = P(1, mi);
+    // lines with fixed ma-orbital p1 and p2, varying mi-orbital
+#pragma unroll
     for (int which = 0; which < 2; ++which) {
-      int pf = which == 0 ? p1 : p2;        // fixed (ma) orbital of the cell
-      int other = which == 0 ? p2 : p1;     // the ma particle in the integral
-      int pu;
-      if (mi == 1) {  // cells (pu, pf)
-        int j = ((int)threadIdx.x - (pf - 1)) & (SB - 1);
-        pu = (j < M) ? j + 1 : 0;
-      } else {        // cells (pf, pu)
-        pu = owned_j(pf, M);
-      }
+      const int pf = which == 0 ? p1 : p2;      // fixed (ma) orbital of the cell
+      const int other = which == 0 ? p2 : p1;   // the ma particle in the integral
+      const int pu = (mi == 1) ? row_of(pf) : owned_j(pf, M);
       if (!pu || lbanned(pu, mi)) continue;
       if (ban_get(BB, M, pf, pu)) continue;
-      double hij = C.I(other, pfix, hfix, pu) * phase_bi<NI>(sel, ma, mi, hfix, other, pu, pfix);
-      if (mi == 1) C.add(pu, pf, coefs, hij);
-      else C.add(pf, pu, coefs, hij);
+      sv[1 + which] = true;
+      sci[1 + which] = (mi == 1) ? pu : pf;
+      scj[1 + which] = (mi == 1) ? pf : pu;
+      sia[1 + which] = idx4(other, pfix, hfix, pu);
+      sib[1 + which] = -1;
+      sg[1 + which] = phase_bi<NI>(sel, ma, mi, hfix, other, pu, pfix);
+    }
+  } else if (np[ma - 1] == 3) {
+    const int hfix = H(1, ma);
+#pragma unroll
+    for (int i = 1; i <= 3; ++i) {
+      const int puti = P(i, ma);
+      const int i1 = (i == 1) ? 2 : 1, i2 = (i == 3) ? 2 : 3;
+      const int p1 = P(i1, ma), p2 = P(i2, ma);
+      // cells {x, puti}: (x, puti) for x < puti, (puti, x) for x >= puti
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        const int k = 2 * (i - 1) + half;
+        int x;
+        if (half == 0) {
+          x = row_of(puti);
+          if (!x || x >= puti) continue;
+        } else {
+          x = owned_j(puti, M);
+          if (!x || x < puti) continue;
+        }
+        const int putj = x;
+        if (putj == hfix) continue;
+        if (lbanned(putj, ma) || ban_get(B1, M, puti, putj)) continue;
+        sv[k] = true;
+        sci[k] = half == 0 ? x : puti;
+        scj[k] = half == 0 ? puti : x;
+        if (putj < hfix) {
+          sia[k] = idx4(p1, p2, putj, hfix);
+          sib[k] = idx4(p2, p1, putj, hfix);
+          sg[k] = phase_bi<NI>(sel, ma, ma, putj, p1, hfix, p2);
+        } else {
+          sia[k] = idx4(p1, p2, hfix, putj);
+          sib[k] = idx4(p2, p1, hfix, putj);
+          sg[k] = phase_bi<NI>(sel, ma, ma, hfix, p1, putj, p2);
+        }
+      }
     }
   } else {
-    if (np[ma - 1] == 3) {
-      int hfix = H(1, ma);
-      for (int i = 1; i <= 3; ++i) {
-        int puti = P(i, ma);
-        int i1 = (i == 1) ? 2 : 1, i2 = (i == 3) ? 2 : 3;
-        int p1 = P(i1, ma), p2 = P(i2, ma);
-        // cells {x, puti}: (x, puti) for x < puti, (puti, x) for x >= puti
-        for (int half = 0; half < 2; ++half) {
-          int x;
-          if (half == 0) {
-            int j = ((int)threadIdx.x - (puti - 1)) & (SB - 1);
-            x = (j < M) ? j + 1 : 0;
-            if (!x || x >= puti) continue;
-          } else {
-            x = owned_j(puti, M);
-            if (!x || x < puti) continue;
-          }
-          int putj = x;
-          if (putj == hfix) continue;
-          if (lbanned(putj, ma) || ban_get(B1, M, puti, putj)) continue;
-          double hij;
-          if (putj < hfix)
-            hij = (C.I(p1, p2, putj, hfix) - C.I(p2, p1, putj, hfix)) * phase_bi<NI>(sel, ma, ma, putj, p1, hfix, p2);
-          else
-            hij = (C.I(p1, p2, hfix, putj) - C.I(p2, p1, hfix, putj)) * phase_bi<NI>(sel, ma, ma, hfix, p1, putj, p2);
-          if (half == 0) C.add(x, puti, coefs, hij);
-          else C.add(puti, x, coefs, hij);
+    const int hfix = H(1, mi), pfix = P(1, mi), p1 = P(1, ma), p2 = P(2, ma);
+#pragma unroll
+    for (int which = 0; which < 2; ++which) {
+      const int pc = which == 0 ? p2 : p1;   // line {x, pc}
+      const int other = which == 0 ? p1 : p2;
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        const int k = 2 * which + half;
+        int x;
+        if (half == 0) {
+          x = row_of(pc);
+          if (!x || x >= pc) continue;
+        } else {
+          x = owned_j(pc, M);
+          if (!x || x < pc) continue;
         }
+        const int puti = x;
+        if (lbanned(puti, ma)) continue;
+        if (ban_get(B1, M, puti, pc)) continue;
+        sv[k] = true;
+        sci[k] = half == 0 ? x : pc;
+        scj[k] = half == 0 ? pc : x;
+        sia[k] = idx4(pfix, other, hfix, puti);
+        sib[k] = -1;
+        sg[k] = phase_bi<NI>(sel, mi, ma, hfix, pfix, puti, other);
       }
-    } else {
-      int hfix = H(1, mi), pfix = P(1, mi), p1 = P(1, ma), p2 = P(2, ma);
-      for (int which = 0; which < 2; ++which) {
-        int pc = which == 0 ? p2 : p1;     // line {x, pc}
-        int other = which == 0 ? p1 : p2;
-        for (int half = 0; half < 2; ++half) {
-          int x;
-          if (half == 0) {
-            int j = ((int)threadIdx.x - (pc - 1)) & (SB - 1);
-            x = (j < M) ? j + 1 : 0;
-            if (!x || x >= pc) continue;
-          } else {
-            x = owned_j(pc, M);
-            if (!x || x < pc) continue;
-          }
-          int puti = x;
-          if (lbanned(puti, ma)) continue;
-          if (ban_get(B1, M, puti, pc)) continue;
-          double hij = C.I(pfix, other, hfix, puti) * phase_bi<NI>(sel, mi, ma, hfix, pfix, puti, other);
-          if (half == 0) C.add(x, pc, coefs, hij);
-          else C.add(pc, x, coefs, hij);
-        }
+    }
+  }
+  {
+    double va[6], vb[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      va[k] = sv[k] ? C.T.V[sia[k]] : 0.0;
+      vb[k] = (sv[k] && sib[k] >= 0) ? C.T.V[sib[k]] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      if (!sv[k]) continue;
+      double hij;
+      if (sib[k] >= 0) {
+        C.gath += 2;
+        hij = (va[k] - vb[k]) * sg[k];
+      } else {
+        C.gath += 1;
+        hij = va[k] * sg[k];
       }
+      C.add(sci[k], scj[k], coefs, hij);
     }
   }
   // singles folded into this block
@@ -662,7 +793,7 @@ __device__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const int (*h)[2], const
     int ib = (s1 == s2) ? i1 + 1 : 1;
     for (int i2 = ib; i2 <= np[s2 - 1]; ++i2) {
       int p1 = P(i1, s1), p2 = P(i2, s2);
-      if (!owns(p1, p2)) continue;
+      if (!owns(p1, p2, M)) continue;
       if (borb_get(BO, p1, s1) || borb_get(BO, p2, s2) || ban_get(B1, M, p1, p2)) continue;
       uint64_t det[2 * NI];
 #pragma unroll
@@ -677,49 +808,101 @@ __device__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const int (*h)[2], const
 }
 
 template <int NI>
-__device__ void get_d0(Ctx<NI> &C, const uint64_t *sel, const int (*p)[4], int sp, const double *coefs, int na_e,
+__device__ __forceinline__ void get_d0(Ctx<NI> &C, const uint64_t *sel, const int (*p)[4], int sp, const double *coefs, int na_e,
                        int nb_e) {
+  // one cell per orbital row and thread.  Cells on the rows and columns of
+  // the selector's own particles need the full matrix element (i_H_j) and
+  // are done separately; for the others the integrals of 8 (N_int = 1) or 4
+  // rows are loaded
+  // together before they are used.
+  constexpr int RB8 = (NI == 1) ? 8 : 4;   // rows per batch of loads
   const int M = C.M;
   const uint32_t *BO = C.L.borb;
   const uint32_t *B1 = C.L.ban1;
-  uint64_t det[2 * NI];
-  if (sp == 3) {
-    int h1 = P(1, 1), h2 = P(1, 2);
-    for (int p1 = 1; p1 <= M; ++p1) {
-      int p2 = owned_j(p1, M);
-      if (!p2) continue;
-      if (borb_get(BO, p1, 1) || borb_get(BO, p2, 2) || ban_get(B1, M, p1, p2)) continue;
-      double hij;
-      if (p1 == h1 || p2 == h2) {
+  auto full = [&](int i, int j, int si, int sj) __attribute__((always_inline)) {   // cell (i, j) via i_H_j
+    uint64_t det[2 * NI];
 #pragma unroll
-        for (int w = 0; w < 2 * NI; ++w) det[w] = C.mask[w];
-        det[(p1 - 1) >> 6] |= 1ULL << ((p1 - 1) & 63);
-        det[NI + ((p2 - 1) >> 6)] |= 1ULL << ((p2 - 1) & 63);
-        hij = i_H_j<NI>(C.T, sel, det, na_e, nb_e);
-        C.gath += 4;
-      } else {
-        hij = C.I(p1, p2, h1, h2) * phase_bi<NI>(sel, 1, 2, h1, p1, h2, p2);
+    for (int w = 0; w < 2 * NI; ++w) det[w] = C.mask[w];
+    det[(si - 1) * NI + ((i - 1) >> 6)] |= 1ULL << ((i - 1) & 63);
+    det[(sj - 1) * NI + ((j - 1) >> 6)] |= 1ULL << ((j - 1) & 63);
+    C.gath += 4;
+    C.add(i, j, coefs, i_H_j<NI>(C.T, sel, det, na_e, nb_e));
+  };
+  auto col_owner_row = [&](int j) -> int { return owned_i(j, M); };   // i with owns(i, j), 0 if none
+  int r_lo, r_hi;
+  band_rows(M, r_lo, r_hi);
+  if (sp == 3) {
+    const int h1 = P(1, 1), h2 = P(1, 2);
+    for (int r0 = r_lo; r0 <= r_hi; r0 += RB8) {
+      int cj[RB8];
+      bool ok[RB8];
+      double v[RB8];
+#pragma unroll
+      for (int b = 0; b < RB8; ++b) {
+        const int p1 = r0 + b;
+        const int p2 = (p1 <= r_hi) ? owned_j(p1, M) : 0;
+        const bool o = p2 && p1 != h1 && p2 != h2 &&
+                       !(borb_get(BO, p1, 1) || borb_get(BO, p2, 2) || ban_get(B1, M, p1, p2));
+        cj[b] = p2;
+        ok[b] = o;
+        v[b] = o ? I4(C.T, p1, p2, h1, h2) : 0.0;
       }
-      C.add(p1, p2, coefs, hij);
+#pragma unroll
+      for (int b = 0; b < RB8; ++b) {
+        if (!ok[b]) continue;
+        const int p1 = r0 + b, p2 = cj[b];
+        C.gath++;
+        C.add(p1, p2, coefs, v[b] * phase_bi<NI>(sel, 1, 2, h1, p1, h2, p2));
+      }
+    }
+    // row h1 and column h2
+    {
+      const int p2 = owned_j(h1, M);
+      if (p2 && !(borb_get(BO, h1, 1) || borb_get(BO, p2, 2) || ban_get(B1, M, h1, p2))) full(h1, p2, 1, 2);
+      const int p1 = col_owner_row(h2);
+      if (p1 && p1 != h1 && !(borb_get(BO, p1, 1) || borb_get(BO, h2, 2) || ban_get(B1, M, p1, h2)))
+        full(p1, h2, 1, 2);
     }
   } else {
-    int p1 = P(1, sp), p2 = P(2, sp);
-    for (int puti = 1; puti <= M; ++puti) {
-      int putj = owned_j(puti, M);
-      if (!putj || putj <= puti) continue;
-      if (borb_get(BO, puti, sp) || borb_get(BO, putj, sp) || ban_get(B1, M, puti, putj)) continue;
-      double hij;
-      if (puti == p1 || putj == p2 || puti == p2 || putj == p1) {
+    const int p1 = P(1, sp), p2 = P(2, sp);
+    for (int r0 = r_lo; r0 <= r_hi; r0 += RB8) {
+      int cj[RB8];
+      bool ok[RB8];
+      double va[RB8], vb[RB8];
 #pragma unroll
-        for (int w = 0; w < 2 * NI; ++w) det[w] = C.mask[w];
-        det[(sp - 1) * NI + ((puti - 1) >> 6)] |= 1ULL << ((puti - 1) & 63);
-        det[(sp - 1) * NI + ((putj - 1) >> 6)] |= 1ULL << ((putj - 1) & 63);
-        hij = i_H_j<NI>(C.T, sel, det, na_e, nb_e);
-        C.gath += 4;
-      } else {
-        hij = (C.I(p1, p2, puti, putj) - C.I(p2, p1, puti, putj)) * phase_bi<NI>(sel, sp, sp, puti, p1, putj, p2);
+      for (int b = 0; b < RB8; ++b) {
+        const int puti = r0 + b;
+        const int putj = (puti <= r_hi) ? owned_j(puti, M) : 0;
+        const bool o = putj && putj > puti && puti != p1 && putj != p2 && puti != p2 && putj != p1 &&
+                       !(borb_get(BO, puti, sp) || borb_get(BO, putj, sp) || ban_get(B1, M, puti, putj));
+        cj[b] = putj;
+        ok[b] = o;
+        va[b] = o ? I4(C.T, p1, p2, puti, putj) : 0.0;
+        vb[b] = o ? I4(C.T, p2, p1, puti, putj) : 0.0;
       }
-      C.add(puti, putj, coefs, hij);
+#pragma unroll
+      for (int b = 0; b < RB8; ++b) {
+        if (!ok[b]) continue;
+        const int puti = r0 + b, putj = cj[b];
+        C.gath += 2;
+        C.add(puti, putj, coefs, (va[b] - vb[b]) * phase_bi<NI>(sel, sp, sp, puti, p1, putj, p2));
+      }
+    }
+    // cells (puti < putj) on the rows and columns of p1 and p2, each once
+    auto try_cell = [&](int i, int j) {
+      if (!(i >= 1 && j > i && j <= M)) return;
+      if (borb_get(BO, i, sp) || borb_get(BO, j, sp) || ban_get(B1, M, i, j)) return;
+      full(i, j, sp, sp);
+    };
+    const int q[2] = {p1, p2};
+    for (int a = 0; a < 2; ++a) {
+      const int o = q[a];
+      // row o: cell (o, j) with j > o
+      const int j = owned_j(o, M);
+      if (j) try_cell(o, j);
+      // column o: cell (i, o) with i < o, unless row i is p1 or p2 (done above)
+      const int i = col_owner_row(o);
+      if (i && i != p1 && i != p2) try_cell(i, o);
     }
   }
 #undef P
@@ -796,13 +979,15 @@ __device__ inline double diag_fock(const Tabs &T, const double *f, int sp, int h
 // One selector in the LDS queue of a block: its index, kind (0: four
 // particles outside the mask, couplings precomputed; 1: three; 2: two or
 // fewer) with the number of precomputed couplings, the couplings' cells
-// (i | j << 8) and values, and its coefficients.
+// (i | j << 8, two per word) and values, its coefficients and determinant.
+template <int NI>
 struct SelQ {
   int idx;
   int kind_n;
-  uint16_t cell[6];
+  uint32_t cellw[3];
   double hij[6];
   double coef[MAXNS];
+  uint64_t det[2 * NI];
 };
 
 template <int NI>
@@ -1013,91 +1198,168 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
   // get_d1 / get_d0 with every thread.  Each cell thus receives its terms in
   // selector order, as in the reference.
   Ctx<NI> C(T, A, L, mask);
-  SelQ *Q = reinterpret_cast<SelQ *>(red);
+  SelQ<NI> *Q = reinterpret_cast<SelQ<NI> *>(red);
   auto flush = [&](int nq) {
-    for (int k = 0; k < nq; ++k) {
-      const SelQ &q = Q[k];
-      const int kind = q.kind_n & 0xf;
-      if (kind == 0) {
-        const int nc = q.kind_n >> 4;
-        for (int c = 0; c < nc; ++c) {
-          const int ci = q.cell[c] & 0xff, cj = q.cell[c] >> 8;
-          if (owns(ci, cj)) C.add(ci, cj, q.coef, q.hij[c]);
+    int k = 0;
+    while (k < nq) {
+      // a run of four-particle entries: headers 8 at a time, then each
+      // thread adds the values of the cells it owns
+      const int m = min(8, nq - k);
+      int kn[8];
+      uint32_t cw[8][3];
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        if (b < m) {
+          kn[b] = Q[k + b].kind_n;
+#pragma unroll
+          for (int w = 0; w < 3; ++w) cw[b][w] = Q[k + b].cellw[w];
+        } else {
+          kn[b] = 1;
         }
-      } else {
-        const uint64_t *d = A.psi + (int64_t)(q.idx - 1) * 2 * NI;
+      }
+      int r = 0;
+#pragma unroll
+      for (int b = 0; b < 8; ++b)
+        if (r == b && (kn[b] & 0xf) == 0) ++r;
+      // this thread's cells among the run, in order; the first two are
+      // handled with their LDS loads issued together
+      int own_n = 0, e0 = 0, e1 = 0;
+      uint32_t c0 = 0, c1 = 0;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        if (b >= r) continue;
+        const int nc = kn[b] >> 4;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          if (c >= nc) continue;
+          const uint32_t wd = (c < 2) ? cw[b][0] : ((c < 4) ? cw[b][1] : cw[b][2]);
+          const uint32_t cell = (wd >> (16 * (c & 1))) & 0xffffu;
+          if (!owns(cell & 0xff, cell >> 8, M)) continue;
+          const int ec = (b << 3) | c;
+          if (own_n == 0) { e0 = ec; c0 = cell; }
+          else if (own_n == 1) { e1 = ec; c1 = cell; }
+          ++own_n;
+        }
+      }
+      if (own_n == 1) {
+        C.add(c0 & 0xff, c0 >> 8, Q[k + (e0 >> 3)].coef, Q[k + (e0 >> 3)].hij[e0 & 7]);
+      } else if (own_n == 2 && c0 != c1) {
+        const SelQ<NI> &q0 = Q[k + (e0 >> 3)], &q1 = Q[k + (e1 >> 3)];
+        const double h0 = q0.hij[e0 & 7], h1 = q1.hij[e1 & 7];
+        double *m0 = L.mat + (int64_t)NS * (((int)(c0 & 0xff) - 1) + M * ((int)(c0 >> 8) - 1));
+        double *m1 = L.mat + (int64_t)NS * (((int)(c1 & 0xff) - 1) + M * ((int)(c1 >> 8) - 1));
+        for (int st = 0; st < NS; ++st) {
+          const double a0 = m0[st], a1 = m1[st], f0 = q0.coef[st], f1 = q1.coef[st];
+          m0[st] = a0 + f0 * h0;
+          m1[st] = a1 + f1 * h1;
+        }
+      } else if (own_n >= 2) {
+        // same cell twice, or more than two cells: in order, one by one
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          if (b >= r) continue;
+          const int nc = kn[b] >> 4;
+          for (int c = 0; c < nc; ++c) {
+            const uint32_t wd = (c < 2) ? cw[b][0] : ((c < 4) ? cw[b][1] : cw[b][2]);
+            const uint32_t cell = (wd >> (16 * (c & 1))) & 0xffffu;
+            const int ci = cell & 0xff, cj = cell >> 8;
+            if (owns(ci, cj, M)) C.add(ci, cj, Q[k + b].coef, Q[k + b].hij[c]);
+          }
+        }
+      }
+      k += r;
+      if (r < m) {
+        const SelQ<NI> &q = Q[k];
+        const int kind = q.kind_n & 0xf;
+        uint64_t d[2 * NI];
+#pragma unroll
+        for (int w = 0; w < 2 * NI; ++w) d[w] = q.det[w];
         int p[2][4], np[2], h[2][2], nh[2];
         sel_lists<NI>(mask, d, p, np, h, nh);
         if (kind == 1) get_d1<NI>(C, d, h, p, nh, np, sp, q.coef, A.na, A.nb);
         else get_d0<NI>(C, d, p, sp, q.coef, A.na, A.nb);
+        ++k;
       }
     }
   };
   const int lane = tid & 63, wv = tid >> 6;
   int qn = 0;
-  for (int x0 = 0; x0 < n_near; x0 += SB) {
-    const int x = x0 + tid;
-    bool pass = false;
-    int idx = 0, nt = 0;
+  // the near-list entry and determinant of the next chunk are loaded while
+  // the current one is processed
+  int nx_idx = 0;
+  uint64_t nx_d[2 * NI];
+  auto fetch = [&](int x) {
+    nx_idx = 0;
     if (x < n_near) {
-      idx = near[x];
-      if (idx >= B.gen && idx <= A.n_sel) {
-        const uint64_t *d = A.psi + (int64_t)(idx - 1) * 2 * NI;
+      nx_idx = near[x];
+      const uint64_t *d = A.psi + (int64_t)(nx_idx - 1) * 2 * NI;
+#pragma unroll
+      for (int w = 0; w < 2 * NI; ++w) nx_d[w] = d[w];
+    }
+  };
+  fetch(tid);
+  // one extra round past the end of the list flushes what is left
+  for (int x0 = 0;; x0 += SB) {
+    const bool more = x0 < n_near;
+    const int idx = nx_idx;
+    uint64_t d[2 * NI];
+#pragma unroll
+    for (int w = 0; w < 2 * NI; ++w) d[w] = nx_d[w];
+    bool pass = false;
+    int nt = 0, cnt = 0, pos = 0;
+    if (more) {
+      fetch(x0 + SB + tid);
+      if (idx >= B.gen && idx <= A.n_sel) {   // idx == 0 past the end of the list
 #pragma unroll
         for (int w = 0; w < 2 * NI; ++w) nt += __popcll(~mask[w] & d[w]);
         pass = nt <= 4;
       }
-    }
-    D2List dl;
-    dl.n = 0;
-    double coefs[MAXNS];
-    if (pass) {
-      const uint64_t *d = A.psi + (int64_t)(idx - 1) * 2 * NI;
-      for (int st = 0; st < NS; ++st) coefs[st] = A.coef[(idx - 1) + A.n_det * st];
-      C.gath += NS;
-      if (nt == 4) {
-        int p[2][4], np[2], h[2][2], nh[2];
-        sel_lists<NI>(mask, d, p, np, h, nh);
-        get_d2<NI>(C, d, h, p, nh, np, sp, dl);
+      // order-preserving compaction: position of this survivor in the chunk
+      const unsigned long long bal = __ballot(pass);
+      const int before = __popcll(bal & ((1ULL << lane) - 1ULL));
+      if (lane == 0) wsum[wv] = __popcll(bal);
+      __syncthreads();
+      int off = 0;
+      for (int w = 0; w < SB / 64; ++w) {
+        if (w < wv) off += wsum[w];
+        cnt += wsum[w];
       }
+      pos = off + before;
     }
-    // order-preserving compaction: position of this survivor in the chunk
-    const unsigned long long bal = __ballot(pass);
-    const int before = __popcll(bal & ((1ULL << lane) - 1ULL));
-    if (lane == 0) wsum[wv] = __popcll(bal);
-    __syncthreads();
-    int off = 0, cnt = 0;
-    for (int w = 0; w < SB / 64; ++w) {
-      if (w < wv) off += wsum[w];
-      cnt += wsum[w];
-    }
-    const int pos = off + before;
     int done = 0;
     while (true) {
       const int space = A.qcap - qn;
       if (pass && pos >= done && pos < done + space) {
-        SelQ &q = Q[qn + pos - done];
+        // the survivor fits in the queue: fill its entry (the four-particle
+        // couplings are computed straight into it)
+        SelQ<NI> &q = Q[qn + pos - done];
         q.idx = idx;
-        q.kind_n = (nt == 4 ? 0 : (nt == 3 ? 1 : 2)) | (dl.n << 4);
-        for (int c = 0; c < dl.n; ++c) {
-          q.cell[c] = (uint16_t)dl.cell[c];
-          q.hij[c] = dl.hij[c];
+        for (int st = 0; st < NS; ++st) q.coef[st] = A.coef[(idx - 1) + A.n_det * st];
+        C.gath += NS;
+#pragma unroll
+        for (int w = 0; w < 2 * NI; ++w) q.det[w] = d[w];
+        D2List dl{q.cellw, q.hij, 0};
+        if (nt == 4) {
+          int p[2][4], np[2], h[2][2], nh[2];
+          sel_lists<NI>(mask, d, p, np, h, nh);
+          get_d2<NI>(C, d, h, p, nh, np, sp, dl);
         }
-        for (int st = 0; st < NS; ++st) q.coef[st] = coefs[st];
+        q.kind_n = (nt == 4 ? 0 : (nt == 3 ? 1 : 2)) | (dl.n << 4);
       }
       const int adv = min(cnt - done, space);
       qn += adv;
       done += adv;
-      if (done == cnt) break;
+      if (done == cnt && more) break;
+      // queue full, or the end of the list: flush
       __syncthreads();
       flush(qn);
       __syncthreads();
       qn = 0;
+      if (done == cnt) break;
     }
+    if (!more) break;
     __syncthreads();
   }
-  flush(qn);
-  __syncthreads();
 
   // ---- fill_buffer_double ----
   const int s1 = (sp == 3) ? 1 : sp, s2 = (sp == 3) ? 2 : sp;
@@ -1105,7 +1367,9 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
   for (int st = 0; st < NS; ++st) pt2[st] = 0.0;
   int64_t ncand = 0;
   bool overflow = false;
-  for (int p1 = 1; p1 <= M; ++p1) {
+  int r_lo, r_hi;
+  band_rows(M, r_lo, r_hi);
+  for (int p1 = r_lo; p1 <= r_hi; ++p1) {
     int p2 = owned_j(p1, M);
     if (!p2) continue;
     if (sp != 3 && p2 <= p1) continue;
@@ -1228,12 +1492,12 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
   // scratch shared with the selector queue
   const size_t lds_base = sizeof(double) * ((size_t)NS * M * M + 2 * (M + 1)) + sizeof(uint32_t) * (2 * nbits + 16) +
                           sizeof(int) * 12 + 8;
-  const size_t red_bytes = sizeof(double) * SB * MAXNS;
+  const size_t red_bytes = sizeof(double) * SB * std::max(NS, 1);
   QPX_REQUIRE(lds_base + red_bytes <= (size_t)c->lds_per_block,
               "selection: N_states * mo_num^2 coupling matrix does not fit in LDS of one workgroup");
-  const int qcap = (int)std::min<size_t>(1024, ((size_t)c->lds_per_block - lds_base) / sizeof(SelQ));
+  const int qcap = (int)std::min<size_t>(1024, ((size_t)c->lds_per_block - lds_base) / sizeof(SelQ<NI>));
   QPX_REQUIRE(qcap >= 16, "selection: no LDS left for the selector queue");
-  const size_t lds = lds_base + std::max(red_bytes, (size_t)qcap * sizeof(SelQ));
+  const size_t lds = lds_base + std::max(red_bytes, (size_t)qcap * sizeof(SelQ<NI>));
   auto kfun = select_block_kernel<NI>;
   QPX_HIP(hipFuncSetAttribute((const void *)kfun, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 
