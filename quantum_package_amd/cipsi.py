@@ -170,6 +170,11 @@ def cipsi(ctx, H, n_det_max: int, n_states: int = 1, n_states_diag: int = 4, s2_
                 e[k] = 1.0
                 Hm[:, k], _ = ctx.H_S2_u_0_nstates(e)
             w, V = np.linalg.eigh(0.5 * (Hm + Hm.T))
+            if n < n_states:
+                # fewer determinants than states: the missing states get
+                # zero vectors (no PT2) until psi has grown
+                w = np.concatenate([w, np.full(n_states - n, w[-1])])
+                V = np.concatenate([V, np.zeros((n, n_states - n))], axis=1)
             return w[:n_states], V[:, :n_states]
         g = np.zeros((n, n_states))
         g[:, :] = coef[:, :n_states]
@@ -193,7 +198,8 @@ def cipsi(ctx, H, n_det_max: int, n_states: int = 1, n_states_diag: int = 4, s2_
         n_gen = n_leading(contrib, threshold_generators)
         n_sel = n_det if threshold_generators >= 1.0 else max(n_leading(contrib, threshold_selectors, True), n_gen)
         ctx.set_wavefunction(dets_s)
-        E0 = np.array([ctx.u_0_H_u_0(coef_s[:, k])[0] for k in range(n_states)])
+        E0 = np.array([ctx.u_0_H_u_0(coef_s[:, k])[0] if np.any(coef_s[:, k]) else energies[k]
+                       for k in range(n_states)])
         to_select = (n_det // 2 + 1) if s2_eig else n_det
         to_select = min(to_select, n_det_max - n_det)
         t1 = time.time()
@@ -213,7 +219,8 @@ def cipsi(ctx, H, n_det_max: int, n_states: int = 1, n_states_diag: int = 4, s2_
                 dets = np.concatenate([dets, extra], axis=0)
                 coef = np.concatenate([coef, np.zeros((extra.shape[0], n_states))], axis=0)
             rec["n_s2_added"] = int(extra.shape[0])
-        coef /= np.linalg.norm(coef, axis=0)
+        nrm = np.linalg.norm(coef, axis=0)
+        coef /= np.where(nrm > 0, nrm, 1.0)
         history.append(rec)
         log(f"cipsi: N_det={n_det} E={energies} PT2={pt2} gen={n_gen} sel={n_sel} -> {dets.shape[0]} "
             f"({rec['t_diag']:.1f}s diag, {rec['t_sel']:.1f}s sel)")

@@ -54,3 +54,23 @@ def test_make_s2_eigenfunction_nothing_to_add():
     d = hf_determinant(8, 3, 3)[None]
     assert make_s2_eigenfunction(d, 3).shape == (0, 2, 1)
     assert det_keys(np.zeros((0, 2, 1), dtype=np.uint64)).shape == (0,)
+
+
+def test_cipsi_two_states_on_oracle():
+    """Two states from HF: the second state has no vector until psi holds
+    two determinants; the final energies respect the variational bound of
+    the full-CI eigenvalues (min-max)."""
+    from test_gpu_cipsi import OracleBackend
+    from quantum_package_amd.cipsi import cipsi
+    from quantum_package_amd.hamiltonian import fci_space, synthetic_hamiltonian
+    from oracle import oracle as O
+    H = synthetic_hamiltonian(8, 3, 3, seed=11)
+    r = cipsi(OracleBackend(), H, n_det_max=400, n_states=2, s2_eig=True, log=lambda m: None)
+    assert r.coef.shape[1] == 2 and np.all(np.isfinite(r.energies))
+    dets = fci_space(8, 3, 3)
+    n = dets.shape[0]
+    ham, bil = O.Ham(H), O.Bilinear(dets)
+    Hm, _ = O.sigma(ham, bil, np.eye(n))
+    w = np.linalg.eigvalsh(0.5 * (Hm + Hm.T))
+    assert r.energies[0] >= w[0] - 1e-10 and r.energies[1] >= w[1] - 1e-10
+    assert r.energies[0] <= r.energies[1]
