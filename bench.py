@@ -23,7 +23,7 @@ The JSON line also carries:
                 selection step from HF, then diagonalised), the leading
                 generators by weight timed on the GPU and, on a smaller
                 generator sample, on the CPU oracle.
-Legs after the first are skipped with --no-davidson / --no-pt2; a leg that
+Legs after the first are skipped with --no-davidson / --no-pt2 / --no-cipsi; a leg that
 fails records its error in its own field and the line is still printed.
 """
 from __future__ import annotations
@@ -67,6 +67,8 @@ def main():
     ap.add_argument("--no-pt2", action="store_true")
     ap.add_argument("--pt2-ndet", type=int, default=100000)
     ap.add_argument("--pt2-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cipsi", action="store_true")
+    ap.add_argument("--cipsi-ndet", type=int, default=32768)
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -203,6 +205,8 @@ def main():
     ctx.close()
     if not args.no_pt2:
         out["pt2"] = _guarded(pt2_leg, args, log)
+    if not args.no_cipsi:
+        out["cipsi_step"] = _guarded(cipsi_step_leg, args, log)
     print(json.dumps(out), flush=True)
 
 
@@ -241,6 +245,76 @@ def davidson_leg(ctx, H, dets, log):
                        "not spin-adapted) + 1 extra vector, threshold 1e-10 on the squared residual, HF + seeded "
                        "random guess, same 9,018,009-determinant space")
     return res
+
+
+def cipsi_step_leg(args, log):
+    """Plain vs learned CIPSI step on a CIPSI-grown psi."""
+    import numpy as np
+    from quantum_package_amd.api import BlockMLP, Context
+    from quantum_package_amd.cipsi import cipsi, make_s2_eigenfunction, n_leading, sort_by_weight
+    from quantum_package_amd.hamiltonian import synthetic_hamiltonian
+    from quantum_package_amd.proposal import Proposer, block_descriptors
+    M, na, nb, seed, ns = 96, 10, 10, 1003, 2
+    H = synthetic_hamiltonian(M, na, nb, seed=seed)
+    ctx = Context(0)
+    mlp = BlockMLP(n_orb=M, seed=7)
+    prop = Proposer(mlp, epochs=20)
+    t0 = time.time()
+    r = cipsi(ctx, H, n_det_max=args.cipsi_ndet, n_states=ns, s2_eig=True, proposer=prop, full_sweep_every=2,
+              log=lambda m: log("bench: " + m))
+    log(f"bench: cipsi grown to {r.dets.shape[0]} dets in {time.time() - t0:.1f}s")
+    ds, cs, contrib = sort_by_weight(r.dets, r.coef)
+    n = ds.shape[0]
+    n_gen = n_leading(contrib, 0.99)
+    n_sel = max(n_leading(contrib, 0.999, True), n_gen)
+    ctx.set_wavefunction(ds)
+    E0 = np.array([ctx.u_0_H_u_0(cs[:, k])[0] for k in range(ns)])
+    cf = np.asfortranarray(cs)
+    to_select = n // 2 + 1
+    # selection: full sweep, then the proposed blocks (network trained on the
+    # loop's last full sweep)
+    t = time.time()
+    pt2_full, new_full, _ = ctx.ZMQ_selection(cf, n_gen, n_sel, E0, to_select)
+    t_full = time.time() - t
+    t = time.time()
+    desc, fdesc = block_descriptors(H, ds, cs, n_gen)
+    keep = prop.propose(desc, fdesc)
+    t_pred = time.time() - t
+    t = time.time()
+    pt2_prop, new_prop, _, _, _ = ctx.select_blocks(cf, n_gen, n_sel, E0, to_select, keep=keep)
+    t_prop = time.time() - t
+    missed = float(1.0 - pt2_prop[0] / pt2_full[0])
+    log(f"bench: selection full {t_full:.2f}s, proposed {t_prop:.2f}s + {t_pred:.2f}s descriptors/prediction, "
+        f"kept {keep.mean():.3f}, missed PT2 {missed:.2e}")
+    # Davidson on the grown psi (the full sweep's determinants), FP64 then mixed
+    dets = np.concatenate([ds, new_full])
+    extra = make_s2_eigenfunction(dets, na)
+    dets = np.concatenate([dets, extra])
+    g = np.zeros((dets.shape[0], ns))
+    g[:n] = cs
+    ctx.set_wavefunction(dets)
+    res = {}
+    for name, mixed in (("fp64", False), ("mixed", True)):
+        t = time.time()
+        e, s2, _, nsig = ctx.davidson_diag_hs2(g, N_st=ns, N_st_diag=4, threshold=1e-10, s2_eig=True, mixed=mixed,
+                                               switch_threshold=1e-6)
+        res[name] = (time.time() - t, e[:ns], nsig)
+        log(f"bench: davidson {name} on {dets.shape[0]} dets: {res[name][0]:.2f}s, E = {e[:ns]}, calls {nsig}")
+    t_plain = res["fp64"][0] + t_full
+    t_learned = res["mixed"][0] + t_prop + t_pred
+    mlp.close()
+    ctx.close()
+    return {"workload": f"one CIPSI step, synthetic Hamiltonian seed {seed}: {M} MOs, {na} alpha + {nb} beta "
+                        f"electrons, N_int=2, {ns} states, s2_eig; psi of {n} determinants grown by the CIPSI "
+                        f"loop (full sweeps every second iteration train the block proposer), selection of "
+                        f"N_det/2+1 = {to_select} with {n_gen} generators and {n_sel} selectors, then Davidson "
+                        f"(threshold 1e-10) on the {dets.shape[0]} determinants of the grown psi",
+            "plain_seconds": t_plain, "learned_seconds": t_learned, "speedup": t_plain / t_learned,
+            "selection_full_seconds": t_full, "selection_proposed_seconds": t_prop,
+            "proposal_overhead_seconds": t_pred, "blocks_kept": float(keep.mean()), "missed_pt2_fraction": missed,
+            "davidson_fp64_seconds": res["fp64"][0], "davidson_mixed_seconds": res["mixed"][0],
+            "davidson_sigma_calls": {"fp64": list(res["fp64"][2]), "mixed": list(res["mixed"][2])},
+            "abs_dE_per_root": np.abs(res["mixed"][1] - res["fp64"][1]).tolist()}
 
 
 def _selected_psi(ctx, H, n_target, log):

@@ -2,7 +2,9 @@
 (plugins/Full_CI_ZMQ/fci_zmq.irp.f:57-170) around the GPU hot path.
 
 Per iteration:
-  1. diagonalize_CI (davidson_diag_hs2 on the current psi);
+  1. diagonalize_CI: dense up to n_det_max_jacobi determinants (the
+     reference's Lapack branch with its S^2 state choice), davidson_diag_hs2
+     above (src/Davidson/diagonalize_CI.irp.f:57-145);
   2. psi_det_sorted: determinants by decreasing state-averaged weight
      (src/Determinants/determinants.irp.f:228-293); generators / selectors
      are the leading determinants up to threshold_generators /
@@ -144,7 +146,8 @@ class CipsiResult:
 def cipsi(ctx, H, n_det_max: int, n_states: int = 1, n_states_diag: int = 4, s2_eig: bool = True,
           threshold_davidson: float = 1e-10, threshold_generators: float = 0.99,
           threshold_selectors: float = 0.999, pt2_max: float = 1e-4, guess_dets=None, log=print,
-          mixed: bool = False, proposer=None, full_sweep_every: int = 2, audit_proposal: bool = False):
+          mixed: bool = False, proposer=None, full_sweep_every: int = 2, audit_proposal: bool = False,
+          n_det_max_jacobi: int = 2000):
     """Run the CIPSI loop on the GPU context ctx from HF (or guess_dets)."""
     from .hamiltonian import hf_determinant
     M, na, nb = H.M, H.n_alpha, H.n_beta
@@ -162,14 +165,29 @@ def cipsi(ctx, H, n_det_max: int, n_states: int = 1, n_states_diag: int = 4, s2_
     def diagonalize(dets, coef):
         ctx.set_wavefunction(dets)
         n = dets.shape[0]
-        if n < 3 * n_states_diag:
-            # small psi: dense diagonalisation (N_det <= N_det_max_jacobi)
+        if n <= n_det_max_jacobi or n < 3 * n_states_diag:
+            # dense diagonalisation, as the reference's "Lapack" branch
+            # (src/Davidson/diagonalize_CI.irp.f:70-145) for N_det <= N_det_max_jacobi
             Hm = np.zeros((n, n))
-            for k in range(n):
-                e = np.zeros(n)
-                e[k] = 1.0
-                Hm[:, k], _ = ctx.H_S2_u_0_nstates(e)
+            Sm = np.zeros((n, n))
+            for k0 in range(0, n, 8):
+                k1 = min(n, k0 + 8)
+                e = np.zeros((n, k1 - k0))
+                e[np.arange(k0, k1), np.arange(k1 - k0)] = 1.0
+                hv, sv = ctx.H_S2_u_0_nstates(e)
+                Hm[:, k0:k1] = hv
+                Sm[:, k0:k1] = sv
             w, V = np.linalg.eigh(0.5 * (Hm + Hm.T))
+            s2v = np.einsum("ik,ij,jk->k", V, 0.5 * (Sm + Sm.T), V)
+            order = np.arange(n)
+            if s2_eig:
+                sz = 0.5 * (na - nb)
+                good = np.abs(s2v - sz * (sz + 1.0)) <= 0.5
+                sel = np.nonzero(good)[0][:n_states]
+                if sel.size:
+                    order = np.concatenate([sel, np.array([j for j in range(n) if j not in set(sel.tolist())],
+                                                          dtype=np.int64)])
+            w, V = w[order], V[:, order]
             if n < n_states:
                 # fewer determinants than states: the missing states get
                 # zero vectors (no PT2) until psi has grown

@@ -462,69 +462,92 @@ __global__ __launch_bounds__(256) void sigma_generic_kernel(GenArgs A, const T *
     for (int st = 0; st < GMAXST; ++st)
       if (st < nst) acc[st] += hij * u[e * nst + st];
   }
-  // alpha/beta doubles
-  if (A.cost_a[a] <= A.cost_b[b]) {
-    for (int es = A.as_ptr[a]; es < A.as_ptr[a + 1]; ++es) {
-      const int ap = A.as_idx[es];
-      const uint32_t ai = A.as_info[es];
-      const int ha = ai & 0xff, pa = (ai >> 8) & 0xff, pha = (ai >> 16) & 1;
-      const int64_t Pa = pair_index(ha, pa);
-      for (int t = A.row_ptr[ap] + lane; t < A.row_ptr[ap + 1]; t += 64) {
+  // alpha/beta doubles.  The determinant's alpha-singles a' (or
+  // beta-singles b') are taken 64 at a time; their rows (columns) are
+  // concatenated and the wavefront's lanes stride over the concatenation,
+  // each lane finding its partner by a binary search of the lanes' prefix
+  // sums (shuffles).  Long and short rows are thus spread evenly.
+  const bool by_rows = A.cost_a[a] <= A.cost_b[b];
+  const int s0 = by_rows ? A.as_ptr[a] : A.bs_ptr[b], s1 = by_rows ? A.as_ptr[a + 1] : A.bs_ptr[b + 1];
+  const int *sidx = by_rows ? A.as_idx : A.bs_idx;
+  const uint32_t *sinfo = by_rows ? A.as_info : A.bs_info;
+  const int *lptr = by_rows ? A.row_ptr : A.col_ptr;
+  for (int es0 = s0; es0 < s1; es0 += 64) {
+    const int es = es0 + lane;
+    int len = 0, start = 0;
+    uint32_t info = 0;
+    if (es < s1) {
+      const int xp = sidx[es];
+      info = sinfo[es];
+      start = lptr[xp];
+      len = lptr[xp + 1] - start;
+    }
+    int pre = len;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(pre, off, 64);
+      if (lane >= off) pre += y;
+    }
+    const int total = __shfl(pre, 63, 64);
+    for (int f0 = 0; f0 < total; f0 += 64) {
+      const int f = f0 + lane;
+      int lo = 0, hi = 63;
+#pragma unroll
+      for (int it = 0; it < 6; ++it) {
+        const int mid = (lo + hi) >> 1;
+        const int pm = __shfl(pre, mid, 64);
+        if (lo < hi) {
+          if (pm > f) hi = mid;
+          else lo = mid + 1;
+        }
+      }
+      const int pre_p = __shfl(pre, lo, 64), len_p = __shfl(len, lo, 64), start_p = __shfl(start, lo, 64);
+      const uint32_t inf = (uint32_t)__shfl((int)info, lo, 64);
+      if (f >= total) continue;
+      const int t = start_p + f - (pre_p - len_p);
+      const int hs = inf & 0xff, ps = (inf >> 8) & 0xff, phs = (inf >> 16) & 1;
+      uint64_t xw[NI];
+      int d = 0;
+      if (by_rows) {
         const int bi = A.csr_b[t];
-        uint64_t xw[NI];
-        int d = 0;
 #pragma unroll
         for (int w = 0; w < NI; ++w) {
           xw[w] = A.bstr[(int64_t)bi * NI + w];
           d += __popcll(bw[w] ^ xw[w]);
         }
-        if (d != 2) continue;
-        int hb, pb;
-        single_hp<NI>(bw, xw, &hb, &pb);
-        const bool neg = pha ^ between_parity<NI>(bw, hb, pb);
-        T hij = V[Pa * npair + pair_index(hb, pb)];
-        if (neg) hij = -hij;
-        const bool flip = (ha == pb) && (pa == hb);
-        const int64_t e = A.csr2csc[t];
-#pragma unroll
-        for (int st = 0; st < GMAXST; ++st)
-          if (st < nst) {
-            T uu = u[e * nst + st];
-            acc[st] += hij * uu;
-            if (flip) accs[st] += neg ? uu : -uu;
-          }
-      }
-    }
-  } else {
-    for (int es = A.bs_ptr[b]; es < A.bs_ptr[b + 1]; ++es) {
-      const int bp = A.bs_idx[es];
-      const uint32_t bi = A.bs_info[es];
-      const int hb = bi & 0xff, pb = (bi >> 8) & 0xff, phb = (bi >> 16) & 1;
-      const int64_t Pb = pair_index(hb, pb);
-      for (int e = A.col_ptr[bp] + lane; e < A.col_ptr[bp + 1]; e += 64) {
-        const int ai = A.csc_a[e];
-        uint64_t xw[NI];
-        int d = 0;
+      } else {
+        const int ai = A.csc_a[t];
 #pragma unroll
         for (int w = 0; w < NI; ++w) {
           xw[w] = A.astr[(int64_t)ai * NI + w];
           d += __popcll(aw[w] ^ xw[w]);
         }
-        if (d != 2) continue;
-        int ha, pa;
-        single_hp<NI>(aw, xw, &ha, &pa);
-        const bool neg = phb ^ between_parity<NI>(aw, ha, pa);
-        T hij = V[pair_index(ha, pa) * npair + Pb];
-        if (neg) hij = -hij;
-        const bool flip = (ha == pb) && (pa == hb);
-#pragma unroll
-        for (int st = 0; st < GMAXST; ++st)
-          if (st < nst) {
-            T uu = u[(int64_t)e * nst + st];
-            acc[st] += hij * uu;
-            if (flip) accs[st] += neg ? uu : -uu;
-          }
       }
+      if (d != 2) continue;
+      int ha, pa, hb, pb;
+      bool neg;
+      int64_t e;
+      if (by_rows) {
+        ha = hs; pa = ps;
+        single_hp<NI>(bw, xw, &hb, &pb);
+        neg = phs ^ between_parity<NI>(bw, hb, pb);
+        e = A.csr2csc[t];
+      } else {
+        hb = hs; pb = ps;
+        single_hp<NI>(aw, xw, &ha, &pa);
+        neg = phs ^ between_parity<NI>(aw, ha, pa);
+        e = t;
+      }
+      T hij = V[pair_index(ha, pa) * npair + pair_index(hb, pb)];
+      if (neg) hij = -hij;
+      const bool flip = (ha == pb) && (pa == hb);
+#pragma unroll
+      for (int st = 0; st < GMAXST; ++st)
+        if (st < nst) {
+          T uu = u[e * nst + st];
+          acc[st] += hij * uu;
+          if (flip) accs[st] += neg ? uu : -uu;
+        }
     }
   }
   // wavefront reduction

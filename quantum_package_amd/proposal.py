@@ -161,11 +161,13 @@ class Proposer:
     """Train on a full sweep (exact block PT2), propose blocks afterwards."""
 
     def __init__(self, mlp, epochs: int = 30, target: float = CALIBRATION_TARGET, lr: float = 1e-3,
-                 batch: int = 2048, seed: int = 0, min_blocks: int = 4096):
+                 batch: int = 2048, seed: int = 0, min_blocks: int = 4096, max_samples: int = 262144):
         self.trainer = Trainer(mlp, lr=lr, batch=batch, seed=seed)
         self.epochs = epochs
         self.target = target
         self.min_blocks = min_blocks
+        self.max_samples = max_samples
+        self.rng = np.random.default_rng(seed + 1)
         self.tau = None
 
     @property
@@ -179,7 +181,12 @@ class Proposer:
         if desc.shape[0] < self.min_blocks:
             return False
         y = labels_from_blocks(block_pt2)
-        self.trainer.fit(desc, fdesc, y, epochs=self.epochs, log=log)
+        # training on a uniform sample of at most max_samples blocks; the
+        # threshold is calibrated on all of them
+        idx = np.arange(desc.shape[0])
+        if idx.size > self.max_samples:
+            idx = np.sort(self.rng.choice(idx.size, self.max_samples, replace=False))
+        self.trainer.fit(desc[idx], fdesc[idx], y[idx], epochs=self.epochs, log=log)
         self.tau = calibrate_threshold(self.trainer.predict(desc, fdesc), block_pt2, self.target)
         return True
 

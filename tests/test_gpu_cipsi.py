@@ -52,8 +52,9 @@ def test_cipsi_gpu_vs_oracle(s2_eig):
     H = synthetic_hamiltonian(12, 4, 4, seed=77)
     _say(f"[cipsi] M=12 4a4b s2_eig={s2_eig}")
     ctx = Context(0)
-    rg = cipsi(ctx, H, n_det_max=600, s2_eig=s2_eig, threshold_davidson=1e-13, log=_say)
-    ro = cipsi(OracleBackend(), H, n_det_max=600, s2_eig=s2_eig, threshold_davidson=1e-13, log=_say)
+    rg = cipsi(ctx, H, n_det_max=600, s2_eig=s2_eig, threshold_davidson=1e-13, log=_say, n_det_max_jacobi=64)
+    ro = cipsi(OracleBackend(), H, n_det_max=600, s2_eig=s2_eig, threshold_davidson=1e-13, log=_say,
+               n_det_max_jacobi=64)
     ctx.close()
     assert len(rg.history) == len(ro.history)
     for a, b in zip(rg.history, ro.history):
