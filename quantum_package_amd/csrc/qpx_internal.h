@@ -88,6 +88,25 @@ struct SpinConn {
   int64_t n_singles = 0, n_doubles = 0;
 };
 
+// Sub-string buckets of the sparse sigma path (sigma_sparse.hip): per
+// excitation class, the sorted entries (determinant | removed orbitals) and,
+// per (determinant, slot), the range of its bucket.
+struct Buckets {
+  int nslot = 0;
+  DBuf<uint64_t> pay;
+  DBuf<int2> range;
+};
+struct SparseIdx {
+  bool valid = false;
+  DBuf<int> a1, a2, b1, b2;   // sub-string ids per unique string
+  Buckets cls[5];             // ab, a1, a2, b1, b2
+  void clear() {
+    valid = false;
+    a1.release(); a2.release(); b1.release(); b2.release();
+    for (auto &b : cls) { b.pay.release(); b.range.release(); b.nslot = 0; }
+  }
+};
+
 struct Wfn {
   int64_t n_det = 0;
   int nint = 0;
@@ -106,6 +125,7 @@ struct Wfn {
   DBuf<int64_t> cost_a, cost_b;
   std::vector<int> col_len_h, row_len_h;
   int max_col = 0, max_row = 0;
+  SparseIdx sp;                           // built on first use of the sparse path
 };
 
 struct Workspace {
@@ -131,8 +151,8 @@ struct qpx_ctx {
   qpx::Workspace ws_sigma_t;    // transposed u_t/v_t/s_t
   qpx::Workspace ws_misc;
   int64_t sel_stats[4] = {0, 0, 0, 0};
-  int sigma_path = 0;           // last sigma path: 1 = dense-map, 2 = generic
-  int force_path = 0;           // 0 auto, 1 dense-map, 2 generic (tests)
+  int sigma_path = 0;           // last sigma path: 1 = dense-map, 2 = generic scan, 3 = sparse buckets
+  int force_path = 0;           // 0 auto, else the path to use (tests)
   int verbosity = 0;            // 1: Davidson iteration table on stderr
   hipEvent_t kev[3] = {nullptr, nullptr, nullptr};  // per-kernel timing (benchmark hook)
   bool ktiming = false;
@@ -143,6 +163,9 @@ namespace qpx {
 void build_wavefunction(qpx_ctx *c, int n_int, int64_t n_det, const uint64_t *psi_det);
 // implemented in sigma.hip
 void sigma_device(qpx_ctx *c, int prec, int n_st, const void *u_t, void *v_t, void *s_t, hipStream_t s);
+// implemented in sigma_sparse.hip
+bool sparse_ok(qpx_ctx *c, int n_st);
+void sigma_sparse(qpx_ctx *c, int prec, int n_st, const void *u_t, void *v_t, void *s_t, hipStream_t s);
 }  // namespace qpx
 
 // Run a body, translating exceptions into return codes.

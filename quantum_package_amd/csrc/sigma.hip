@@ -694,8 +694,14 @@ void sigma_device(qpx_ctx *c, int prec, int nst, const void *u, void *v, void *s
   QPX_REQUIRE(prec == QPX_FP64 || prec == QPX_FP32, "prec must be QPX_FP64 or QPX_FP32");
   size_t ts = prec == QPX_FP64 ? sizeof(double) : sizeof(float);
   bool dense = dense_ok(c, nst, ts);
-  if (c->force_path == 2) dense = false;
+  if (c->force_path >= 2) dense = false;
   if (c->force_path == 1) QPX_REQUIRE(dense, "dense-map sigma path not applicable to this wavefunction");
+  if (!dense && c->force_path != 2 && sparse_ok(c, nst)) {
+    c->sigma_path = 3;
+    sigma_sparse(c, prec, nst, u, v, s, st);
+    return;
+  }
+  QPX_REQUIRE(c->force_path != 3, "sparse sigma path not applicable to this wavefunction");
   const Wfn &W = c->W;
   const int maxs = W.ca.max_singles;
   if (dense) {
@@ -786,7 +792,7 @@ extern "C" int qpx_sigma_device(qpx_ctx *c, int prec, int nst, const void *u_t, 
 // test hook: 0 = automatic, 1 = dense-map only, 2 = generic only
 extern "C" int qpx_debug_force_sigma_path(qpx_ctx *c, int path, int *last_path) {
   return qpx_guard(c, [&] {
-    QPX_REQUIRE(path >= 0 && path <= 2, "path must be 0, 1 or 2");
+    QPX_REQUIRE(path >= 0 && path <= 3, "path must be 0 (automatic), 1 (dense map), 2 (generic scan) or 3 (sparse buckets)");
     c->force_path = path;
     if (last_path) *last_path = c->sigma_path;
   });

@@ -213,6 +213,7 @@ void build_wavefunction(qpx_ctx *c, int nint, int64_t n_det, const uint64_t *psi
   QPX_REQUIRE(psi != nullptr, "null psi_det");
   Wfn &W = c->W;
   W.valid = false;
+  W.sp.clear();
   W.n_det = n_det;
   W.nint = nint;
   hipStream_t s = c->stream;

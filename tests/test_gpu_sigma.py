@@ -59,11 +59,12 @@ CASES = [
     ("sparse-M12", 12, 4, 4, 3000, 7),
     ("nint2-M70", 70, 3, 3, 2500, 8),
     ("nint2-M96", 96, 5, 5, 2000, 9),
+    ("fci-M12-3a1b", 12, 3, 1, None, 10),
 ]
 
 
 @pytest.mark.parametrize("name,M,na,nb,n,seed", CASES)
-@pytest.mark.parametrize("path", [0, 1, 2])
+@pytest.mark.parametrize("path", [0, 1, 2, 3])
 def test_sigma_fp64_vs_oracle(ctx, name, M, na, nb, n, seed, path):
     H = synthetic_hamiltonian(M, na, nb, seed=seed)
     dets = fci_space(M, na, nb) if n is None else _random_space(M, na, nb, n, seed)
@@ -94,15 +95,18 @@ def test_sigma_fp64_vs_oracle(ctx, name, M, na, nb, n, seed, path):
     assert np.max(np.abs(s - so)) < 1e-11 * max(1.0, np.max(np.abs(so)))
 
 
+@pytest.mark.parametrize("path", [0, 3])
 @pytest.mark.parametrize("name,M,na,nb,n,seed", CASES[:1] + CASES[3:5])
-def test_sigma_fp32(ctx, name, M, na, nb, n, seed):
+def test_sigma_fp32(ctx, name, M, na, nb, n, seed, path):
     H = synthetic_hamiltonian(M, na, nb, seed=seed)
     dets = fci_space(M, na, nb) if n is None else _random_space(M, na, nb, n, seed)
-    _say(f"[sigma fp32] case={name} n_det={dets.shape[0]}")
+    _say(f"[sigma fp32] case={name} path={path} n_det={dets.shape[0]}")
     ctx.set_hamiltonian(H)
     ctx.set_wavefunction(dets)
     u = np.random.default_rng(3).standard_normal((dets.shape[0], 2))
+    ctx._force_sigma_path(path)
     v, s = ctx.H_S2_u_0_nstates(u, prec=1)
+    ctx._force_sigma_path(0)
     ham = O.Ham(H)
     vo, so = O.sigma(ham, O.Bilinear(dets), u)
     for k in range(2):
