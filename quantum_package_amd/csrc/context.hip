@@ -13,7 +13,7 @@
 
 using namespace qpx;
 
-static inline int64_t pair0(int p, int q) {  // 0-based orbitals
+__host__ __device__ static inline int64_t pair0(int p, int q) {  // 0-based orbitals
   int lo = p < q ? p : q, hi = p < q ? q : p;
   return (int64_t)hi * (hi + 1) / 2 + lo;
 }
@@ -27,6 +27,26 @@ __global__ void expand_pair_matrix(const double *__restrict__ eri, double *__res
     double v = eri[hi * (hi + 1) / 2 + lo];
     V[x] = v;
     Vf[x] = (float)v;
+  }
+}
+
+// single-excitation tables: for the pair (h, p) and every orbital k,
+// GS[(h*M+p)*M+k] = (hp|kk) - (hk|kp) (k of the excited spin) and
+// GO[(h*M+p)*M+k] = (hp|kk) (k of the other spin), so that a single
+// excitation element is h_hp + sum over the two strings' occupations
+__global__ void single_tables_kernel(const double *__restrict__ V, int M, int npair, double *__restrict__ GS,
+                                     double *__restrict__ GO, float *__restrict__ GSf, float *__restrict__ GOf) {
+  const int64_t n = (int64_t)M * M * M;
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(x % M);
+    const int64_t hp = x / M;
+    const int h = (int)(hp / M), p = (int)(hp % M);
+    const double go = V[pair0(h, p) * npair + pair0(k, k)];
+    const double gs = go - V[pair0(h, k) * npair + pair0(k, p)];
+    GS[x] = gs;
+    GO[x] = go;
+    GSf[x] = (float)gs;
+    GOf[x] = (float)go;
   }
 }
 
@@ -159,6 +179,14 @@ extern "C" int qpx_set_hamiltonian(qpx_ctx *c, int M, int na, int nb, const doub
     int grid = (int)std::min<int64_t>((npair * npair + 255) / 256, 65536);
     hipLaunchKernelGGL(expand_pair_matrix, dim3(grid), dim3(256), 0, c->stream, eri_d.p, H.V.p, H.Vf.p,
                        (int)npair);
+    QPX_HIP(hipGetLastError());
+    const size_t m3 = (size_t)M * M * M;
+    H.GS.alloc(m3);
+    H.GO.alloc(m3);
+    H.GSf.alloc(m3);
+    H.GOf.alloc(m3);
+    hipLaunchKernelGGL(single_tables_kernel, dim3((int)std::min<size_t>((m3 + 255) / 256, 65536)), dim3(256), 0,
+                       c->stream, H.V.p, M, (int)npair, H.GS.p, H.GO.p, H.GSf.p, H.GOf.p);
     QPX_HIP(hipGetLastError());
     QPX_HIP(hipStreamSynchronize(c->stream));
     c->ham_set = true;

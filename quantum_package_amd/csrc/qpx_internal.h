@@ -72,6 +72,8 @@ struct Ham {
   DBuf<double> JAd;               // M*M: J_ij - K_ij (mo_bielec_integral_jj_anti)
   DBuf<double> fock_cs;           // M*M fock_operator_closed_shell_ref_bitmask
   DBuf<int> ptab;                 // M*M pair index
+  DBuf<double> GS, GO;            // M^3 single-excitation tables (context.hip)
+  DBuf<float> GSf, GOf;
   double ref_energy = 0.0;        // ref_bitmask_energy of the HF determinant
 };
 

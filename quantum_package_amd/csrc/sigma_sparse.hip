@@ -191,24 +191,26 @@ __global__ void bucket_finalize_kernel(ClassArgs C, const int *__restrict__ orig
   }
 }
 
+// single excitation h -> p of the string `same` (other spin `other`):
+// h_hp + sum_{k in same} GS[hp][k] + sum_{k in other} GO[hp][k]
 template <typename T, int NI>
-__device__ inline T single_elem_sp(const uint64_t *same, const uint64_t *other, int h, int p, const T *V,
-                                   const T *hcore, int M, int npair) {
-  const int64_t P = pair_index(h, p);
+__device__ inline T single_elem_sp(const uint64_t *same, const uint64_t *other, int h, int p, const T *GS,
+                                   const T *GO, const T *hcore, int M) {
+  const int64_t row = ((int64_t)h * M + p) * M;
   T e = hcore[h * M + p];
 #pragma unroll
   for (int w = 0; w < NI; ++w) {
     uint64_t y = same[w];
     while (y) {
-      int k = 64 * w + __ffsll((unsigned long long)y) - 1;
+      const int k = 64 * w + __ffsll((unsigned long long)y) - 1;
       y &= y - 1;
-      e += V[P * npair + pair_index(k, k)] - V[pair_index(h, k) * npair + pair_index(k, p)];
+      e += GS[row + k];
     }
     y = other[w];
     while (y) {
-      int k = 64 * w + __ffsll((unsigned long long)y) - 1;
+      const int k = 64 * w + __ffsll((unsigned long long)y) - 1;
       y &= y - 1;
-      e += V[P * npair + pair_index(k, k)];
+      e += GO[row + k];
     }
   }
   return e;
@@ -227,6 +229,7 @@ struct SparseArgs {
 
 template <typename T, int NI>
 __global__ __launch_bounds__(256) void sigma_sparse_kernel(SparseArgs A, const T *__restrict__ V,
+                                                           const T *__restrict__ GS, const T *__restrict__ GO,
                                                            const T *__restrict__ hcore, const T *__restrict__ u,
                                                            T *__restrict__ v, T *__restrict__ s) {
   __shared__ uint8_t occ_s[4][2][SP_MAXE];
@@ -272,6 +275,9 @@ __global__ __launch_bounds__(256) void sigma_sparse_kernel(SparseArgs A, const T
       if (lane >= off) pre += y;
     }
     const int total = __shfl(pre, 63, 64);
+    // flat index f of slot p maps to entry f + off_p of its bucket
+    const int off = beg - (pre - len);
+    const int csl = (cls << 24) | sl;
     for (int f0 = 0; f0 < total; f0 += 64) {
       const int f = f0 + lane;
       int lo = 0, hi = 63;
@@ -284,10 +290,10 @@ __global__ __launch_bounds__(256) void sigma_sparse_kernel(SparseArgs A, const T
           else lo = mid + 1;
         }
       }
-      const int pre_p = __shfl(pre, lo, 64), len_p = __shfl(len, lo, 64), beg_p = __shfl(beg, lo, 64);
-      const int cls_p = __shfl(cls, lo, 64), sl_p = __shfl(sl, lo, 64);
+      const int off_p = __shfl(off, lo, 64), csl_p = __shfl(csl, lo, 64);
+      const int cls_p = csl_p >> 24, sl_p = csl_p & 0xffffff;
       if (f >= total) continue;
-      const int e = beg_p + f - (pre_p - len_p);
+      const int e = f + off_p;
       const uint64_t py = A.pay[cls_p][e];
       const int64_t d2 = (int64_t)(uint32_t)py;
       const int x = (int)((py >> 32) & 0xff), y = (int)((py >> 40) & 0xff);
@@ -300,22 +306,33 @@ __global__ __launch_bounds__(256) void sigma_sparse_kernel(SparseArgs A, const T
         neg = between_parity<NI>(aw, i, x) ^ between_parity<NI>(bw, j, y);
         hij = V[pair_index(i, x) * npair + pair_index(j, y)];
         flip = (i == y) && (x == j);
-      } else if (cls_p == 1 || cls_p == 3) {
-        const uint64_t *same = cls_p == 1 ? aw : bw, *oth = cls_p == 1 ? bw : aw;
-        const int i = occ_s[wv][cls_p == 1 ? 0 : 1][sl_p];
+      } else if (cls_p == 1) {
+        // the spin strings are passed through explicit branches: choosing a
+        // pointer between the two private arrays gave wrong beta-double
+        // phases in the FP32 instantiation
+        const int i = occ_s[wv][0][sl_p];
         if (x == i) continue;
-        hij = single_elem_sp<T, NI>(same, oth, i, x, V, hcore, M, npair);
-        neg = between_parity<NI>(same, i, x);
+        hij = single_elem_sp<T, NI>(aw, bw, i, x, GS, GO, hcore, M);
+        neg = between_parity<NI>(aw, i, x);
+      } else if (cls_p == 3) {
+        const int i = occ_s[wv][1][sl_p];
+        if (x == i) continue;
+        hij = single_elem_sp<T, NI>(bw, aw, i, x, GS, GO, hcore, M);
+        neg = between_parity<NI>(bw, i, x);
       } else {
-        const int sp = cls_p == 2 ? 0 : 1;
-        const int ne = sp == 0 ? A.ne_a : A.ne_b;
-        int ii, jj;
-        pair_of(sl_p, ne, &ii, &jj);
-        const int h1 = occ_s[wv][sp][ii], h2 = occ_s[wv][sp][jj];
+        int ii, jj, h1, h2;
+        if (cls_p == 2) {
+          pair_of(sl_p, A.ne_a, &ii, &jj);
+          h1 = occ_s[wv][0][ii];
+          h2 = occ_s[wv][0][jj];
+        } else {
+          pair_of(sl_p, A.ne_b, &ii, &jj);
+          h1 = occ_s[wv][1][ii];
+          h2 = occ_s[wv][1][jj];
+        }
         if (x == h1 || x == h2 || y == h1 || y == h2) continue;
-        const uint64_t *same = sp == 0 ? aw : bw;
         hij = V[pair_index(h1, x) * npair + pair_index(h2, y)] - V[pair_index(h1, y) * npair + pair_index(h2, x)];
-        neg = double_parity<NI>(same, h1, h2, x, y);
+        neg = (cls_p == 2) ? double_parity<NI>(aw, h1, h2, x, y) : double_parity<NI>(bw, h1, h2, x, y);
       }
       if (neg) hij = -hij;
 #pragma unroll
@@ -438,10 +455,14 @@ static void launch_sparse(qpx_ctx *c, int nst, const T *u, T *v, T *s, hipStream
   A.n_det = W.n_det; A.M = c->H.M; A.npair = c->H.npair; A.nst = nst; A.ne_a = c->H.na; A.ne_b = c->H.nb;
   const int64_t blocks = (W.n_det + 3) / 4;
   QPX_REQUIRE(blocks < (1LL << 31), "too many determinants for one launch");
-  const T *V = std::is_same<T, double>::value ? (const T *)c->H.V.p : (const T *)c->H.Vf.p;
-  const T *hc = std::is_same<T, double>::value ? (const T *)c->H.hcore.p : (const T *)c->H.hcoref.p;
+  const bool f64 = std::is_same<T, double>::value;
+  const T *V = f64 ? (const T *)c->H.V.p : (const T *)c->H.Vf.p;
+  const T *hc = f64 ? (const T *)c->H.hcore.p : (const T *)c->H.hcoref.p;
+  const T *GS = f64 ? (const T *)c->H.GS.p : (const T *)c->H.GSf.p;
+  const T *GO = f64 ? (const T *)c->H.GO.p : (const T *)c->H.GOf.p;
   if (c->ktiming) QPX_HIP(hipEventRecord(c->kev[0], st));
-  hipLaunchKernelGGL((sigma_sparse_kernel<T, NI>), dim3((unsigned)blocks), dim3(256), 0, st, A, V, hc, u, v, s);
+  hipLaunchKernelGGL((sigma_sparse_kernel<T, NI>), dim3((unsigned)blocks), dim3(256), 0, st, A, V, GS, GO, hc, u, v,
+                     s);
   QPX_HIP(hipGetLastError());
   if (c->ktiming) {
     QPX_HIP(hipEventRecord(c->kev[1], st));
