@@ -224,10 +224,10 @@ struct SparseArgs {
   const uint64_t *pay[5];
   int nslot[5];
   int64_t n_det;
-  int M, npair, nst, ne_a, ne_b;
+  int M, npair, ne_a, ne_b;
 };
 
-template <typename T, int NI>
+template <typename T, int NI, int NST>
 __global__ __launch_bounds__(256) void sigma_sparse_kernel(SparseArgs A, const T *__restrict__ V,
                                                            const T *__restrict__ GS, const T *__restrict__ GO,
                                                            const T *__restrict__ hcore, const T *__restrict__ u,
@@ -236,7 +236,8 @@ __global__ __launch_bounds__(256) void sigma_sparse_kernel(SparseArgs A, const T
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int64_t k = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
   if (k >= A.n_det) return;   // whole wavefront
-  const int nst = A.nst, M = A.M, npair = A.npair;
+  constexpr int nst = NST;
+  const int M = A.M, npair = A.npair;
   const int a = A.csc_a[k], b = A.csc_b[k];
   uint64_t aw[NI], bw[NI];
 #pragma unroll
@@ -249,9 +250,9 @@ __global__ __launch_bounds__(256) void sigma_sparse_kernel(SparseArgs A, const T
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  T acc[SP_MAXST], accs[SP_MAXST];
+  T acc[NST], accs[NST];
 #pragma unroll
-  for (int st = 0; st < SP_MAXST; ++st) acc[st] = accs[st] = T(0);
+  for (int st = 0; st < NST; ++st) acc[st] = accs[st] = T(0);
 
   int cum[6];
   cum[0] = 0;
@@ -335,18 +336,18 @@ __global__ __launch_bounds__(256) void sigma_sparse_kernel(SparseArgs A, const T
         neg = (cls_p == 2) ? double_parity<NI>(aw, h1, h2, x, y) : double_parity<NI>(bw, h1, h2, x, y);
       }
       if (neg) hij = -hij;
+      T uu[NST];
 #pragma unroll
-      for (int st = 0; st < SP_MAXST; ++st)
-        if (st < nst) {
-          const T uu = u[d2 * nst + st];
-          acc[st] += hij * uu;
-          if (flip) accs[st] += neg ? uu : -uu;
-        }
+      for (int st = 0; st < NST; ++st) uu[st] = u[d2 * NST + st];
+#pragma unroll
+      for (int st = 0; st < NST; ++st) {
+        acc[st] += hij * uu[st];
+        if (flip) accs[st] += neg ? uu[st] : -uu[st];
+      }
     }
   }
 #pragma unroll
-  for (int st = 0; st < SP_MAXST; ++st) {
-    if (st >= nst) break;
+  for (int st = 0; st < NST; ++st) {
     T xx = acc[st], yy = accs[st];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -359,12 +360,11 @@ __global__ __launch_bounds__(256) void sigma_sparse_kernel(SparseArgs A, const T
   if (lane == 0) {
     const T hd = (T)A.hdiag[k], sd = (T)A.sdiag[k];
 #pragma unroll
-    for (int st = 0; st < SP_MAXST; ++st)
-      if (st < nst) {
-        const T uu = u[k * nst + st];
-        v[k * nst + st] = acc[st] + hd * uu;
-        s[k * nst + st] = accs[st] + sd * uu;
-      }
+    for (int st = 0; st < NST; ++st) {
+      const T uu = u[k * NST + st];
+      v[k * NST + st] = acc[st] + hd * uu;
+      s[k * NST + st] = accs[st] + sd * uu;
+    }
   }
 }
 
@@ -452,7 +452,7 @@ static void launch_sparse(qpx_ctx *c, int nst, const T *u, T *v, T *s, hipStream
     A.pay[q] = W.sp.cls[q].pay.p;
     A.nslot[q] = W.sp.cls[q].nslot;
   }
-  A.n_det = W.n_det; A.M = c->H.M; A.npair = c->H.npair; A.nst = nst; A.ne_a = c->H.na; A.ne_b = c->H.nb;
+  A.n_det = W.n_det; A.M = c->H.M; A.npair = c->H.npair; A.ne_a = c->H.na; A.ne_b = c->H.nb;
   const int64_t blocks = (W.n_det + 3) / 4;
   QPX_REQUIRE(blocks < (1LL << 31), "too many determinants for one launch");
   const bool f64 = std::is_same<T, double>::value;
@@ -461,8 +461,17 @@ static void launch_sparse(qpx_ctx *c, int nst, const T *u, T *v, T *s, hipStream
   const T *GS = f64 ? (const T *)c->H.GS.p : (const T *)c->H.GSf.p;
   const T *GO = f64 ? (const T *)c->H.GO.p : (const T *)c->H.GOf.p;
   if (c->ktiming) QPX_HIP(hipEventRecord(c->kev[0], st));
-  hipLaunchKernelGGL((sigma_sparse_kernel<T, NI>), dim3((unsigned)blocks), dim3(256), 0, st, A, V, GS, GO, hc, u, v,
-                     s);
+  const dim3 g((unsigned)blocks), bl(256);
+  switch (nst) {
+    case 1: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 1>), g, bl, 0, st, A, V, GS, GO, hc, u, v, s); break;
+    case 2: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 2>), g, bl, 0, st, A, V, GS, GO, hc, u, v, s); break;
+    case 3: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 3>), g, bl, 0, st, A, V, GS, GO, hc, u, v, s); break;
+    case 4: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 4>), g, bl, 0, st, A, V, GS, GO, hc, u, v, s); break;
+    case 5: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 5>), g, bl, 0, st, A, V, GS, GO, hc, u, v, s); break;
+    case 6: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 6>), g, bl, 0, st, A, V, GS, GO, hc, u, v, s); break;
+    case 7: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 7>), g, bl, 0, st, A, V, GS, GO, hc, u, v, s); break;
+    default: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 8>), g, bl, 0, st, A, V, GS, GO, hc, u, v, s); break;
+  }
   QPX_HIP(hipGetLastError());
   if (c->ktiming) {
     QPX_HIP(hipEventRecord(c->kev[1], st));
