@@ -67,6 +67,11 @@ def main():
     ap.add_argument("--no-pt2", action="store_true")
     ap.add_argument("--pt2-ndet", type=int, default=100000)
     ap.add_argument("--pt2-seconds", type=float, default=20.0)
+    ap.add_argument("--no-selected", action="store_true")
+    ap.add_argument("--sel-small", type=int, default=8000,
+                    help="CIPSI loop size before the large selection step of the selected-psi leg")
+    ap.add_argument("--sel-select", type=int, default=930000,
+                    help="determinants selected in one step before the S^2 completion (about 2^21 after it)")
     ap.add_argument("--no-cipsi", action="store_true")
     ap.add_argument("--cipsi-ndet", type=int, default=32768)
     ap.add_argument("--calib-copy", action="store_true",
@@ -203,6 +208,8 @@ def main():
     if not args.no_davidson:
         out["davidson"] = _guarded(davidson_leg, ctx, H, dets, log)
     ctx.close()
+    if not args.no_selected:
+        out["selected_davidson"] = _guarded(selected_davidson_leg, args, log)
     if not args.no_pt2:
         out["pt2"] = _guarded(pt2_leg, args, log)
     if not args.no_cipsi:
@@ -244,6 +251,135 @@ def davidson_leg(ctx, H, dets, log):
     res["workload"] = ("davidson_diag_hs2, lowest 3 roots of H (no S^2 filter: seeded random guess vectors are "
                        "not spin-adapted) + 1 extra vector, threshold 1e-10 on the squared residual, HF + seeded "
                        "random guess, same 9,018,009-determinant space")
+    return res
+
+
+def sparse_alg_bytes(n_det, visited, na, nb, nst, w):
+    """Algorithmic bytes of one sparse sigma call.  From the visited-entry
+    counts of the five bucket classes: S_a = V_a1 - N na ordered alpha-single
+    pairs (S_b likewise); a pair of alpha singles shares n_b ab buckets and
+    n_a - 1 aa buckets, a determinant meets itself in every slot, so the
+    connected pairs are ab = V_ab - N na nb - S_b na - S_a nb and
+    aa = V_aa - N na(na-1)/2 - S_a (na-1).  Bytes: 8 per visited payload, 8 per
+    (determinant, slot) range, an nst*w row of u per connected pair, w per
+    ab integral, 2w per same-spin double, (na+nb+1) w per single, and the
+    determinant's own u, v, s rows."""
+    V = visited
+    sa = V["a1"] - n_det * na
+    sb = V["b1"] - n_det * nb
+    ab = V["ab"] - n_det * na * nb - sb * na - sa * nb
+    aa = V["a2"] - n_det * na * (na - 1) // 2 - sa * (na - 1)
+    bb = V["b2"] - n_det * nb * (nb - 1) // 2 - sb * (nb - 1)
+    slots = na * nb + na + nb + na * (na - 1) // 2 + nb * (nb - 1) // 2
+    pairs = ab + aa + bb + sa + sb
+    b = (8 * sum(V.values()) + 8 * n_det * slots + pairs * nst * w + ab * w + 2 * (aa + bb) * w
+         + (sa + sb) * (na + nb + 1) * w + 3 * n_det * nst * w)
+    return b, {"ab": ab, "aa": aa, "bb": bb, "a": sa, "b": sb}
+
+
+def selected_davidson_leg(args, log):
+    """Mixed-precision vs FP64 davidson_diag_hs2 on a selected wavefunction
+    of about 2^21 determinants (sparse sigma path), sigma kernel timings and
+    the CPU oracle on a sample of rows."""
+    import ctypes
+    import numpy as np
+    import torch
+    from quantum_package_amd.api import FP32, FP64, Context
+    from quantum_package_amd.cipsi import cipsi, make_s2_eigenfunction, n_leading, sort_by_weight
+    from quantum_package_amd.hamiltonian import synthetic_hamiltonian
+    M, na, nb, seed, ns, nst = 96, 10, 10, 1003, 2, 4
+    H = synthetic_hamiltonian(M, na, nb, seed=seed)
+    ctx = Context(0)
+    t0 = time.time()
+    r = cipsi(ctx, H, n_det_max=args.sel_small, n_states=ns, s2_eig=True, log=lambda m: None)
+    ds, cs, contrib = sort_by_weight(r.dets, r.coef)
+    n = ds.shape[0]
+    n_gen = n_leading(contrib, 0.99)
+    n_sel = max(n_leading(contrib, 0.999, True), n_gen)
+    ctx.set_wavefunction(ds)
+    E0 = np.array([ctx.u_0_H_u_0(cs[:, k])[0] for k in range(ns)])
+    _, new, _ = ctx.ZMQ_selection(np.asfortranarray(cs), n_gen, n_sel, E0, args.sel_select)
+    dets = np.concatenate([ds, new])
+    dets = np.concatenate([dets, make_s2_eigenfunction(dets, na)])
+    nd = dets.shape[0]
+    t_build = time.time() - t0
+    log(f"bench: selected psi {nd} dets ({n} by the CIPSI loop, {new.shape[0]} selected in one step, rest S^2 "
+        f"completion) in {t_build:.1f}s")
+    ctx.set_wavefunction(dets)
+    t0 = time.time()
+    visited = ctx._sparse_stats()
+    t_buckets = time.time() - t0
+    path = ctx._force_sigma_path(0)
+    res = {"n_det": nd, "psi_build_seconds": t_build, "bucket_build_seconds": t_buckets, "visited": visited}
+    # sigma kernel timings, 4 vectors, FP64 and FP32
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    kern = {}
+    for prec, dt, w, name in ((FP64, torch.float64, 8, "fp64"), (FP32, torch.float32, 4, "fp32")):
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(11)
+        u = torch.randn(nd * nst, dtype=dt, device=dev, generator=gen)
+        v = torch.empty_like(u)
+        s = torch.empty_like(u)
+        torch.cuda.synchronize()
+        kms = np.zeros(2, dtype=np.float32)
+        ctx._check(ctx.L.qpx_debug_sigma_kernel_times(ctx.ptr, prec, nst, u.data_ptr(), v.data_ptr(), s.data_ptr(),
+                                                      stream.cuda_stream,
+                                                      kms.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 5))
+        b, pairs = sparse_alg_bytes(nd, visited, na, nb, nst, w)
+        ms = float(kms[0])
+        kern[name] = {"kernel_ms": ms, "dets_per_s": nd / (ms * 1e-3),
+                      "roofline": {"bound": "hbm", "achieved": b / (ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
+                                   "unit": "GB/s", "frac": b / (ms * 1e-3) / HBM_PEAK, "alg_bytes": b}}
+        log(f"bench: sparse sigma {name} {nst} vectors: {ms:.2f} ms ({nd / (ms * 1e-3):.3e} dets/s)")
+        del u, v, s
+    res["sigma_sparse"] = kern
+    res["connected_pairs"] = pairs
+    res["sigma_path"] = path
+    g = np.zeros((nd, ns))
+    g[:n] = cs
+    dav = {}
+    for name, mixed in (("fp64", False), ("mixed", True)):
+        t = time.time()
+        e, s2, _, nsig = ctx.davidson_diag_hs2(g, N_st=ns, N_st_diag=4, threshold=1e-10, s2_eig=True, mixed=mixed,
+                                               switch_threshold=1e-6)
+        dav[name] = {"seconds": time.time() - t, "energies": e[:ns].tolist(), "s2": s2[:ns].tolist(),
+                     "sigma_calls_fp64": nsig[0], "sigma_calls_fp32": nsig[1]}
+        log(f"bench: selected davidson {name}: {dav[name]['seconds']:.2f}s, E = {e[:ns]}, calls {nsig}")
+    res["davidson"] = dav
+    res["abs_dE_per_root"] = np.abs(np.array(dav["mixed"]["energies"]) - np.array(dav["fp64"]["energies"])).tolist()
+    res["speedup_mixed"] = dav["fp64"]["seconds"] / dav["mixed"]["seconds"]
+    if not args.no_cpu:
+        from oracle import oracle as O
+        ham = O.Ham(H)
+        bil = O.Bilinear(dets)
+        nthr = args.cpu_threads
+        # calibrate on growing strided samples until one takes over a second
+        # (the first call pays the thread start-up), then run ~cpu_seconds
+        istep = max(1, nd // 256)
+        while True:
+            t0 = time.time()
+            rows, _ = O.sigma_work_sample(ham, bil, nst, istep, nthr)
+            dt0 = time.time() - t0
+            if dt0 > 1.0 or istep == 1:
+                break
+            istep = max(1, istep // 4)
+        rate0 = rows / max(dt0, 1e-6)
+        istep = max(1, nd // max(int(rate0 * args.cpu_seconds), 1))
+        t0 = time.time()
+        rows, _ = O.sigma_work_sample(ham, bil, nst, istep, nthr)
+        dt1 = time.time() - t0
+        res["cpu_baseline"] = {"value": rows / dt1, "unit": "dets/s", "cores": nthr, "kind": "port",
+                               "sample": f"{rows} of {nd} rows (every {istep}-th, bilinear order) of one FP64 "
+                                         f"{nst}-vector sigma call, reference algorithm restated in C/OpenMP; "
+                                         f"{dt1:.1f} s"}
+        log(f"bench: selected psi cpu oracle {rows / dt1:.1f} dets/s on {nthr} threads")
+    res["workload"] = (f"davidson_diag_hs2 on a selected wavefunction, synthetic Hamiltonian seed {seed}: {M} MOs, "
+                       f"{na} alpha + {nb} beta electrons, N_int=2, {nd} determinants (CIPSI loop from HF to "
+                       f"{n}, one selection step of {new.shape[0]}, S^2 completion); {ns} roots + 2 extra vectors, "
+                       f"s2_eig, threshold 1e-10; mixed = FP32 sigma for correction vectors until the residual "
+                       f"falls under 1e-6, then FP64")
+    ctx.close()
     return res
 
 

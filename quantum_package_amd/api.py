@@ -195,6 +195,13 @@ class Context:
         self._check(self.L.qpx_debug_force_sigma_path(self.ptr, path, C.byref(last)))
         return last.value
 
+    def _sparse_stats(self):
+        """Entries the sparse sigma walk visits per class (sum of squared
+        bucket sizes): ab, a1, a2, b1, b2."""
+        v = np.zeros(5, dtype=np.int64)
+        self._check(self.L.qpx_debug_sparse_stats(self.ptr, v.ctypes.data_as(_lib.c_i64p)))
+        return dict(zip(("ab", "a1", "a2", "b1", "b2"), (int(x) for x in v)))
+
 
 # ---------------------------------------------------------------------------
 # block-proposal network (BF16 MFMA kernels in libqpx)

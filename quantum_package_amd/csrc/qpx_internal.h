@@ -95,6 +95,7 @@ struct SpinConn {
 // per (determinant, slot), the range of its bucket.
 struct Buckets {
   int nslot = 0;
+  int64_t visited = 0;   // sum of squared bucket sizes
   DBuf<uint64_t> pay;
   DBuf<int2> range;
 };

@@ -164,6 +164,23 @@ __global__ void bucket_starts_kernel(const int *__restrict__ head, const int *__
     if (head[p]) start[sid[p] - 1] = (int)p;
 }
 
+// per-block partial sums of len^2 over the buckets: the number of entries
+// the walk visits for this class (every member visits its whole bucket)
+__global__ void bucket_visits_kernel(const int *__restrict__ start, int nseg, int64_t n,
+                                     long long *__restrict__ partial) {
+  __shared__ long long red[4];
+  long long acc = 0;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nseg; q += (int64_t)gridDim.x * blockDim.x) {
+    const long long len = (q + 1 < nseg ? (int64_t)start[q + 1] : n) - start[q];
+    acc += len * len;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
 // payload of sorted position p: determinant | removed orbital(s) << 32 /
 // << 40; range of the (determinant, slot) that sits at p
 template <int NI>
@@ -375,6 +392,7 @@ static void build_class(qpx_ctx *c, Buckets &Bk, ClassArgs C) {
   const int64_t n = W.n_det * C.nslot;
   QPX_REQUIRE(n < ((int64_t)1 << 31), "sparse sigma: too many (determinant, slot) entries");
   Bk.nslot = C.nslot;
+  Bk.visited = 0;
   if (n == 0) return;   // e.g. no same-spin doubles with one electron of a spin
   DBuf<uint64_t> key, key2;
   DBuf<int> orig, orig2, head, sid, start;
@@ -407,6 +425,18 @@ static void build_class(qpx_ctx *c, Buckets &Bk, ClassArgs C) {
   start.alloc(nseg);
   hipLaunchKernelGGL(bucket_starts_kernel, dim3(grid), dim3(256), 0, st, head.p, sid.p, n, start.p);
   QPX_HIP(hipGetLastError());
+  {
+    const int vg = (int)std::min<int64_t>((nseg + 255) / 256, 1024);
+    DBuf<long long> part;
+    part.alloc(vg);
+    hipLaunchKernelGGL(bucket_visits_kernel, dim3(vg), dim3(256), 0, st, start.p, nseg, n, part.p);
+    QPX_HIP(hipGetLastError());
+    std::vector<long long> h(vg);
+    QPX_HIP(hipMemcpyAsync(h.data(), part.p, vg * sizeof(long long), hipMemcpyDeviceToHost, st));
+    QPX_HIP(hipStreamSynchronize(st));
+    Bk.visited = 0;
+    for (long long x : h) Bk.visited += x;
+  }
   key2.release();
   Bk.pay.alloc(n);
   Bk.range.alloc(n);
@@ -517,3 +547,24 @@ void sigma_sparse(qpx_ctx *c, int prec, int nst, const void *u, void *v, void *s
 }
 
 }  // namespace qpx
+
+// benchmark hook: entries the sparse walk visits per class (ab, a1, a2, b1,
+// b2) for the current wavefunction; builds the buckets if needed
+extern "C" int qpx_debug_sparse_stats(qpx_ctx *c, int64_t *visited) {
+  return qpx_guard(c, [&] {
+    QPX_REQUIRE(visited, "null output");
+    QPX_REQUIRE(c->ham_set && c->W.valid, "Hamiltonian and wavefunction must be set");
+    QPX_HIP(hipSetDevice(c->device));
+    QPX_REQUIRE(qpx::sparse_ok(c, 1), "the sparse path does not cover this system");
+    Wfn &W = c->W;
+    if (!W.sp.valid) {
+      switch (W.nint) {
+        case 1: build_buckets<1>(c); break;
+        case 2: build_buckets<2>(c); break;
+        case 3: build_buckets<3>(c); break;
+        default: build_buckets<4>(c); break;
+      }
+    }
+    for (int q = 0; q < 5; ++q) visited[q] = W.sp.cls[q].visited;
+  });
+}
