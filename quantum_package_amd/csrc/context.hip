@@ -189,6 +189,8 @@ extern "C" int qpx_set_hamiltonian(qpx_ctx *c, int M, int na, int nb, const doub
                        c->stream, H.V.p, M, (int)npair, H.GS.p, H.GO.p, H.GSf.p, H.GOf.p);
     QPX_HIP(hipGetLastError());
     QPX_HIP(hipStreamSynchronize(c->stream));
+    H.G4.release();   // rebuilt for the new integrals on the next sparse call
+    H.G4f.release();
     c->ham_set = true;
     c->W.valid = false;
   });

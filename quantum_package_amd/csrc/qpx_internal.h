@@ -74,6 +74,10 @@ struct Ham {
   DBuf<int> ptab;                 // M*M pair index
   DBuf<double> GS, GO;            // M^3 single-excitation tables (context.hip)
   DBuf<float> GSf, GOf;
+  // M^4 table G4[((i*M + j)*M + y)*M + x] = (ix|jy), built by the sparse
+  // sigma path on first use when M <= kG4MaxOrb (sigma_sparse.hip)
+  DBuf<double> G4;
+  DBuf<float> G4f;
   double ref_energy = 0.0;        // ref_bitmask_energy of the HF determinant
 };
 

@@ -47,6 +47,29 @@ namespace {
 
 constexpr int SP_MAXE = 32;    // electrons per spin handled by this path
 constexpr int SP_MAXST = 8;
+// largest orbital count for which the M^4 double-excitation table is built
+// (8 + 4 bytes per entry: 7.9 GB at 160 orbitals); above it the kernel reads
+// the pair matrix
+constexpr int kG4MaxOrb = 160;
+
+// G4[((i*M + j)*M + y)*M + x] = (ix|jy): a bucket lists its partners in
+// determinant order, i.e. beta string then alpha string, so consecutive
+// lanes read the same (i, j, y) and neighbouring x -- one cache line for
+// many partners instead of one line each in the pair matrix
+__global__ void g4_fill_kernel(const double *__restrict__ V, int M, int npair, double *__restrict__ G,
+                               float *__restrict__ Gf) {
+  const int64_t n = (int64_t)M * M * M * M;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(q % M);
+    int64_t t = q / M;
+    const int y = (int)(t % M);
+    t /= M;
+    const int j = (int)(t % M), i = (int)(t / M);
+    const double v = V[pair_index(i, x) * npair + pair_index(j, y)];
+    G[q] = v;
+    Gf[q] = (float)v;
+  }
+}
 
 // ids of the (ne - drop)-electron sub-strings of n strings: ids[s * nslot +
 // slot], slot = index of the removed orbital (drop 1) or of the removed pair
@@ -246,12 +269,19 @@ struct SparseArgs {
 
 template <typename T, int NI, int NST>
 __global__ __launch_bounds__(256) void sigma_sparse_kernel(SparseArgs A, const T *__restrict__ V,
-                                                           const T *__restrict__ GS, const T *__restrict__ GO,
-                                                           const T *__restrict__ hcore, const T *__restrict__ u,
-                                                           T *__restrict__ v, T *__restrict__ s) {
+                                                           const T *__restrict__ G4, const T *__restrict__ GS,
+                                                           const T *__restrict__ GO, const T *__restrict__ hcore,
+                                                           const T *__restrict__ u, T *__restrict__ v,
+                                                           T *__restrict__ s) {
   __shared__ uint8_t occ_s[4][2][SP_MAXE];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int64_t k = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
+  // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs; give
+  // each XCD a contiguous run of determinants, whose partners overlap, so
+  // that they share that XCD's L2
+  const int nblk = (int)gridDim.x, base = nblk >> 3, rem = nblk & 7;
+  const int xcd = (int)(blockIdx.x & 7);
+  const int logical = xcd * base + min(xcd, rem) + (int)(blockIdx.x >> 3);
+  const int64_t k = (int64_t)logical * (blockDim.x >> 6) + wv;
   if (k >= A.n_det) return;   // whole wavefront
   constexpr int nst = NST;
   const int M = A.M, npair = A.npair;
@@ -322,7 +352,7 @@ __global__ __launch_bounds__(256) void sigma_sparse_kernel(SparseArgs A, const T
         if (x == i || y == j) continue;
         // alpha i -> x, beta j -> y
         neg = between_parity<NI>(aw, i, x) ^ between_parity<NI>(bw, j, y);
-        hij = V[pair_index(i, x) * npair + pair_index(j, y)];
+        hij = G4 ? G4[((i * M + j) * M + y) * M + x] : V[pair_index(i, x) * npair + pair_index(j, y)];
         flip = (i == y) && (x == j);
       } else if (cls_p == 1) {
         // the spin strings are passed through explicit branches: choosing a
@@ -349,7 +379,10 @@ __global__ __launch_bounds__(256) void sigma_sparse_kernel(SparseArgs A, const T
           h2 = occ_s[wv][1][jj];
         }
         if (x == h1 || x == h2 || y == h1 || y == h2) continue;
-        hij = V[pair_index(h1, x) * npair + pair_index(h2, y)] - V[pair_index(h1, y) * npair + pair_index(h2, x)];
+        // (h1 x|h2 y) - (h1 y|h2 x), the second as (h2 x|h1 y) so that both
+        // reads run along x
+        hij = G4 ? G4[((h1 * M + h2) * M + y) * M + x] - G4[((h2 * M + h1) * M + y) * M + x]
+                 : V[pair_index(h1, x) * npair + pair_index(h2, y)] - V[pair_index(h1, y) * npair + pair_index(h2, x)];
         neg = (cls_p == 2) ? double_parity<NI>(aw, h1, h2, x, y) : double_parity<NI>(bw, h1, h2, x, y);
       }
       if (neg) hij = -hij;
@@ -490,17 +523,18 @@ static void launch_sparse(qpx_ctx *c, int nst, const T *u, T *v, T *s, hipStream
   const T *hc = f64 ? (const T *)c->H.hcore.p : (const T *)c->H.hcoref.p;
   const T *GS = f64 ? (const T *)c->H.GS.p : (const T *)c->H.GSf.p;
   const T *GO = f64 ? (const T *)c->H.GO.p : (const T *)c->H.GOf.p;
+  const T *G4 = f64 ? (const T *)c->H.G4.p : (const T *)c->H.G4f.p;   // null above kG4MaxOrb
   if (c->ktiming) QPX_HIP(hipEventRecord(c->kev[0], st));
   const dim3 g((unsigned)blocks), bl(256);
   switch (nst) {
-    case 1: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 1>), g, bl, 0, st, A, V, GS, GO, hc, u, v, s); break;
-    case 2: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 2>), g, bl, 0, st, A, V, GS, GO, hc, u, v, s); break;
-    case 3: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 3>), g, bl, 0, st, A, V, GS, GO, hc, u, v, s); break;
-    case 4: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 4>), g, bl, 0, st, A, V, GS, GO, hc, u, v, s); break;
-    case 5: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 5>), g, bl, 0, st, A, V, GS, GO, hc, u, v, s); break;
-    case 6: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 6>), g, bl, 0, st, A, V, GS, GO, hc, u, v, s); break;
-    case 7: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 7>), g, bl, 0, st, A, V, GS, GO, hc, u, v, s); break;
-    default: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 8>), g, bl, 0, st, A, V, GS, GO, hc, u, v, s); break;
+    case 1: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 1>), g, bl, 0, st, A, V, G4, GS, GO, hc, u, v, s); break;
+    case 2: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 2>), g, bl, 0, st, A, V, G4, GS, GO, hc, u, v, s); break;
+    case 3: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 3>), g, bl, 0, st, A, V, G4, GS, GO, hc, u, v, s); break;
+    case 4: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 4>), g, bl, 0, st, A, V, G4, GS, GO, hc, u, v, s); break;
+    case 5: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 5>), g, bl, 0, st, A, V, G4, GS, GO, hc, u, v, s); break;
+    case 6: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 6>), g, bl, 0, st, A, V, G4, GS, GO, hc, u, v, s); break;
+    case 7: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 7>), g, bl, 0, st, A, V, G4, GS, GO, hc, u, v, s); break;
+    default: hipLaunchKernelGGL((sigma_sparse_kernel<T, NI, 8>), g, bl, 0, st, A, V, G4, GS, GO, hc, u, v, s); break;
   }
   QPX_HIP(hipGetLastError());
   if (c->ktiming) {
@@ -521,6 +555,16 @@ bool sparse_ok(qpx_ctx *c, int nst) {
 
 void sigma_sparse(qpx_ctx *c, int prec, int nst, const void *u, void *v, void *s, hipStream_t st) {
   Wfn &W = c->W;
+  Ham &H = c->H;
+  if (H.M <= kG4MaxOrb && !H.G4.p) {
+    const int64_t n4 = (int64_t)H.M * H.M * H.M * H.M;
+    H.G4.alloc(n4);
+    H.G4f.alloc(n4);
+    hipLaunchKernelGGL(g4_fill_kernel, dim3((unsigned)std::min<int64_t>((n4 + 255) / 256, 1 << 16)), dim3(256), 0,
+                       c->stream, H.V.p, H.M, H.npair, H.G4.p, H.G4f.p);
+    QPX_HIP(hipGetLastError());
+    QPX_HIP(hipStreamSynchronize(c->stream));
+  }
   if (!W.sp.valid) {
     switch (W.nint) {
       case 1: build_buckets<1>(c); break;
