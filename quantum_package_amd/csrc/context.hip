@@ -50,6 +50,40 @@ __global__ void single_tables_kernel(const double *__restrict__ V, int M, int np
   }
 }
 
+// G4[((i*M + j)*M + y)*M + x] = (ix|jy), 0-based orbitals: every integral
+// (ab|cd) sits in it with any one chosen orbital in the contiguous x slot,
+// so a kernel whose lanes vary one orbital reads consecutive addresses
+__global__ void g4_fill_kernel(const double *__restrict__ V, int M, int npair, double *__restrict__ G,
+                               float *__restrict__ Gf) {
+  const int64_t n = (int64_t)M * M * M * M;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(q % M);
+    int64_t t = q / M;
+    const int y = (int)(t % M);
+    t /= M;
+    const int j = (int)(t % M), i = (int)(t / M);
+    const double v = V[pair0(i, x) * npair + pair0(j, y)];
+    G[q] = v;
+    Gf[q] = (float)v;
+  }
+}
+
+namespace qpx {
+bool ensure_g4(qpx_ctx *c) {
+  Ham &H = c->H;
+  if (H.M > kG4MaxOrb) return false;
+  if (H.G4.p) return true;
+  const int64_t n4 = (int64_t)H.M * H.M * H.M * H.M;
+  H.G4.alloc(n4);
+  H.G4f.alloc(n4);
+  hipLaunchKernelGGL(g4_fill_kernel, dim3((unsigned)std::min<int64_t>((n4 + 255) / 256, 1 << 16)), dim3(256), 0,
+                     c->stream, H.V.p, H.M, H.npair, H.G4.p, H.G4f.p);
+  QPX_HIP(hipGetLastError());
+  QPX_HIP(hipStreamSynchronize(c->stream));
+  return true;
+}
+}  // namespace qpx
+
 extern "C" int qpx_create(int device, qpx_ctx **out) {
   if (!out) return 1;
   *out = nullptr;

@@ -28,6 +28,10 @@ struct Error : std::runtime_error {
   } while (0)
 
 constexpr int NINT_MAX = 4;
+// largest orbital count for which the M^4 integral table (Ham::G4) is built:
+// 8 + 4 bytes per entry, 7.9 GB at 160 orbitals; above it the kernels read
+// the pair matrix
+constexpr int kG4MaxOrb = 160;
 
 // Device buffer with RAII.
 template <typename T>
@@ -74,8 +78,8 @@ struct Ham {
   DBuf<int> ptab;                 // M*M pair index
   DBuf<double> GS, GO;            // M^3 single-excitation tables (context.hip)
   DBuf<float> GSf, GOf;
-  // M^4 table G4[((i*M + j)*M + y)*M + x] = (ix|jy), built by the sparse
-  // sigma path on first use when M <= kG4MaxOrb (sigma_sparse.hip)
+  // M^4 table G4[((i*M + j)*M + y)*M + x] = (ix|jy), built on first use by
+  // the sparse sigma path when M <= kG4MaxOrb (ensure_g4)
   DBuf<double> G4;
   DBuf<float> G4f;
   double ref_energy = 0.0;        // ref_bitmask_energy of the HF determinant
@@ -166,6 +170,9 @@ struct qpx_ctx {
 };
 
 namespace qpx {
+// implemented in context.hip: builds Ham::G4/G4f if M <= kG4MaxOrb and
+// returns whether the table exists
+bool ensure_g4(qpx_ctx *c);
 // implemented in wavefunction.hip
 void build_wavefunction(qpx_ctx *c, int n_int, int64_t n_det, const uint64_t *psi_det);
 // implemented in sigma.hip

@@ -47,30 +47,6 @@ namespace {
 
 constexpr int SP_MAXE = 32;    // electrons per spin handled by this path
 constexpr int SP_MAXST = 8;
-// largest orbital count for which the M^4 double-excitation table is built
-// (8 + 4 bytes per entry: 7.9 GB at 160 orbitals); above it the kernel reads
-// the pair matrix
-constexpr int kG4MaxOrb = 160;
-
-// G4[((i*M + j)*M + y)*M + x] = (ix|jy): a bucket lists its partners in
-// determinant order, i.e. beta string then alpha string, so consecutive
-// lanes read the same (i, j, y) and neighbouring x -- one cache line for
-// many partners instead of one line each in the pair matrix
-__global__ void g4_fill_kernel(const double *__restrict__ V, int M, int npair, double *__restrict__ G,
-                               float *__restrict__ Gf) {
-  const int64_t n = (int64_t)M * M * M * M;
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-    const int x = (int)(q % M);
-    int64_t t = q / M;
-    const int y = (int)(t % M);
-    t /= M;
-    const int j = (int)(t % M), i = (int)(t / M);
-    const double v = V[pair_index(i, x) * npair + pair_index(j, y)];
-    G[q] = v;
-    Gf[q] = (float)v;
-  }
-}
-
 // ids of the (ne - drop)-electron sub-strings of n strings: ids[s * nslot +
 // slot], slot = index of the removed orbital (drop 1) or of the removed pair
 // i < j in lexicographic order over the occupied list (drop 2)
@@ -523,7 +499,10 @@ static void launch_sparse(qpx_ctx *c, int nst, const T *u, T *v, T *s, hipStream
   const T *hc = f64 ? (const T *)c->H.hcore.p : (const T *)c->H.hcoref.p;
   const T *GS = f64 ? (const T *)c->H.GS.p : (const T *)c->H.GSf.p;
   const T *GO = f64 ? (const T *)c->H.GO.p : (const T *)c->H.GOf.p;
-  const T *G4 = f64 ? (const T *)c->H.G4.p : (const T *)c->H.G4f.p;   // null above kG4MaxOrb
+  // G4[((i*M + j)*M + y)*M + x] = (ix|jy); a bucket lists its partners in
+  // determinant order (beta string, then alpha string), so consecutive
+  // lanes share (i, j, y) and read neighbouring x.  Null above kG4MaxOrb.
+  const T *G4 = f64 ? (const T *)c->H.G4.p : (const T *)c->H.G4f.p;
   if (c->ktiming) QPX_HIP(hipEventRecord(c->kev[0], st));
   const dim3 g((unsigned)blocks), bl(256);
   switch (nst) {
@@ -555,16 +534,7 @@ bool sparse_ok(qpx_ctx *c, int nst) {
 
 void sigma_sparse(qpx_ctx *c, int prec, int nst, const void *u, void *v, void *s, hipStream_t st) {
   Wfn &W = c->W;
-  Ham &H = c->H;
-  if (H.M <= kG4MaxOrb && !H.G4.p) {
-    const int64_t n4 = (int64_t)H.M * H.M * H.M * H.M;
-    H.G4.alloc(n4);
-    H.G4f.alloc(n4);
-    hipLaunchKernelGGL(g4_fill_kernel, dim3((unsigned)std::min<int64_t>((n4 + 255) / 256, 1 << 16)), dim3(256), 0,
-                       c->stream, H.V.p, H.M, H.npair, H.G4.p, H.G4f.p);
-    QPX_HIP(hipGetLastError());
-    QPX_HIP(hipStreamSynchronize(c->stream));
-  }
+  ensure_g4(c);   // partners in a bucket then read neighbouring integrals
   if (!W.sp.valid) {
     switch (W.nint) {
       case 1: build_buckets<1>(c); break;
