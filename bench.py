@@ -72,6 +72,8 @@ def main():
                     help="CIPSI loop size before the large selection step of the selected-psi leg")
     ap.add_argument("--sel-select", type=int, default=930000,
                     help="determinants selected in one step before the S^2 completion (about 2^21 after it)")
+    ap.add_argument("--sel-no-davidson", action="store_true",
+                    help="selected-psi leg: sigma timings only (for counter runs)")
     ap.add_argument("--no-cipsi", action="store_true")
     ap.add_argument("--cipsi-ndet", type=int, default=32768)
     ap.add_argument("--calib-copy", action="store_true",
@@ -311,7 +313,16 @@ def selected_davidson_leg(args, log):
     t_buckets = time.time() - t0
     path = ctx._force_sigma_path(0)
     res = {"n_det": nd, "psi_build_seconds": t_build, "bucket_build_seconds": t_buckets, "visited": visited}
-    # sigma kernel timings, 4 vectors, FP64 and FP32
+    # sigma kernel timings, 4 vectors, FP64 and FP32; measured HBM bytes per
+    # call (rocprofv3 FETCH_SIZE/WRITE_SIZE of this leg, calibrated) when the
+    # committed summary is for this psi
+    traffic = {}
+    tpath = os.path.join(ROOT, "profiles", "sigma_sparse_selected_m96_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        if f"{nd:,}" in tj.get("workload", ""):
+            traffic = tj["hbm_bytes_per_sigma"]
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream(device=dev)
     kern = {}
@@ -330,7 +341,8 @@ def selected_davidson_leg(args, log):
         ms = float(kms[0])
         kern[name] = {"kernel_ms": ms, "dets_per_s": nd / (ms * 1e-3),
                       "roofline": {"bound": "hbm", "achieved": b / (ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
-                                   "unit": "GB/s", "frac": b / (ms * 1e-3) / HBM_PEAK, "alg_bytes": b}}
+                                   "unit": "GB/s", "frac": b / (ms * 1e-3) / HBM_PEAK, "alg_bytes": b,
+                                   "traffic": traffic.get(name)}}
         log(f"bench: sparse sigma {name} {nst} vectors: {ms:.2f} ms ({nd / (ms * 1e-3):.3e} dets/s)")
         del u, v, s
     res["sigma_sparse"] = kern
@@ -339,16 +351,18 @@ def selected_davidson_leg(args, log):
     g = np.zeros((nd, ns))
     g[:n] = cs
     dav = {}
-    for name, mixed in (("fp64", False), ("mixed", True)):
+    for name, mixed in (() if args.sel_no_davidson else (("fp64", False), ("mixed", True))):
         t = time.time()
         e, s2, _, nsig = ctx.davidson_diag_hs2(g, N_st=ns, N_st_diag=4, threshold=1e-10, s2_eig=True, mixed=mixed,
                                                switch_threshold=1e-6)
         dav[name] = {"seconds": time.time() - t, "energies": e[:ns].tolist(), "s2": s2[:ns].tolist(),
                      "sigma_calls_fp64": nsig[0], "sigma_calls_fp32": nsig[1]}
         log(f"bench: selected davidson {name}: {dav[name]['seconds']:.2f}s, E = {e[:ns]}, calls {nsig}")
-    res["davidson"] = dav
-    res["abs_dE_per_root"] = np.abs(np.array(dav["mixed"]["energies"]) - np.array(dav["fp64"]["energies"])).tolist()
-    res["speedup_mixed"] = dav["fp64"]["seconds"] / dav["mixed"]["seconds"]
+    if dav:
+        res["davidson"] = dav
+        res["abs_dE_per_root"] = np.abs(np.array(dav["mixed"]["energies"]) -
+                                        np.array(dav["fp64"]["energies"])).tolist()
+        res["speedup_mixed"] = dav["fp64"]["seconds"] / dav["mixed"]["seconds"]
     if not args.no_cpu:
         from oracle import oracle as O
         ham = O.Ham(H)
