@@ -195,6 +195,13 @@ class Context:
         self._check(self.L.qpx_debug_force_sigma_path(self.ptr, path, C.byref(last)))
         return last.value
 
+    def _select_block_ticks(self, n_blocks: int):
+        """Per-block workgroup durations (100 MHz ticks) of the last selection
+        call, in block enumeration order."""
+        t = np.zeros(n_blocks, dtype=np.int64)
+        self._check(self.L.qpx_debug_select_block_ticks(self.ptr, t.ctypes.data_as(_lib.c_i64p), n_blocks))
+        return t
+
     def _sparse_stats(self):
         """Entries the sparse sigma walk visits per class (sum of squared
         bucket sizes): ab, a1, a2, b1, b2."""

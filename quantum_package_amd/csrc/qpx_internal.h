@@ -162,6 +162,7 @@ struct qpx_ctx {
   qpx::Workspace ws_sigma_t;    // transposed u_t/v_t/s_t
   qpx::Workspace ws_misc;
   int64_t sel_stats[4] = {0, 0, 0, 0};
+  std::vector<int64_t> sel_ticks;   // per-block durations of the last selection (benchmark hook)
   int sigma_path = 0;           // last sigma path: 1 = dense-map, 2 = generic scan, 3 = sparse buckets
   int force_path = 0;           // 0 auto, else the path to use (tests)
   int verbosity = 0;            // 1: Davidson iteration table on stderr

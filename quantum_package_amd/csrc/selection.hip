@@ -395,6 +395,7 @@ struct SelArgs {
   int *blk_status;     // 0 done, 1 skipped (owned by an earlier generator), 2 overflow
   int64_t *blk_ncand;
   int64_t *blk_ngath;
+  int64_t *blk_ticks;  // duration of the workgroup in 100 MHz wall-clock ticks (cost model of the proposal)
 };
 
 template <int NI>
@@ -1007,6 +1008,7 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
   double *red = reinterpret_cast<double *>(wsum + 8);
   red = reinterpret_cast<double *>((reinterpret_cast<uintptr_t>(red) + 7) & ~uintptr_t(7));
 
+  const uint64_t t_start = wall_clock64();
   const BlockDesc B = A.blocks[blockIdx.x];
   const int tid = threadIdx.x;
   const uint64_t *gen = A.psi + (int64_t)(B.gen - 1) * 2 * NI;
@@ -1141,6 +1143,7 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
       for (int st = 0; st < NS; ++st) A.blk_pt2[(int64_t)blockIdx.x * NS + st] = 0.0;
       A.blk_ncand[blockIdx.x] = 0;
       A.blk_ngath[blockIdx.x] = 0;
+      A.blk_ticks[blockIdx.x] = (int64_t)(wall_clock64() - t_start);
     }
     return;
   }
@@ -1442,6 +1445,7 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
   if (tid == 0) {
     A.blk_ngath[blockIdx.x] = ired[0];
     A.blk_status[blockIdx.x] = L.flag[1] ? 2 : 0;
+    A.blk_ticks[blockIdx.x] = (int64_t)(wall_clock64() - t_start);
   }
 }
 
@@ -1566,6 +1570,7 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
 
   std::vector<double> blk_pt2_all((size_t)n_blocks_total * NS, 0.0);
   std::vector<int> blk_done(n_blocks_total, 0);
+  std::vector<int64_t> blk_ticks_all(n_blocks_total, 0);
   int64_t ncand_total = 0, ngath_total = 0, nproc = 0;
 
   DBuf<int> gens_d, near_cnt_d, near_list_d;
@@ -1573,7 +1578,7 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
   DBuf<BlockDesc> blocks_d;
   DBuf<double> bpt2_d;
   DBuf<int> bstat_d;
-  DBuf<int64_t> bncand_d, bngath_d;
+  DBuf<int64_t> bncand_d, bngath_d, bticks_d;
   DBuf<uint8_t> drop_d;
   std::vector<uint8_t> drop(n_blocks_total, 0);
 
@@ -1665,6 +1670,7 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
       bstat_d.alloc(nb);
       bncand_d.alloc(nb);
       bngath_d.alloc(nb);
+      bticks_d.alloc(nb);
       // global ids for the pool: the run index maps back through pending
       SelArgs SA{};
       SA.psi = W.psi.p; SA.coef = coef_d.p; SA.n_det = n_det; SA.n_sel = n_sel; SA.NS = NS;
@@ -1675,6 +1681,7 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
       SA.pool_key = P.key.p; SA.pool_det = P.det.p; SA.pool_blk = P.blk.p; SA.pool_count = P.count.p;
       SA.pool_cap = P.cap; SA.tau = tau;
       SA.blk_pt2 = bpt2_d.p; SA.blk_status = bstat_d.p; SA.blk_ncand = bncand_d.p; SA.blk_ngath = bngath_d.p;
+      SA.blk_ticks = bticks_d.p;
       // the pool records run-local block ids; they are translated to global
       // ids right after the launch
       SA.blk_base = 0;
@@ -1687,11 +1694,12 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
       QPX_HIP(hipGetLastError());
       std::vector<double> bp((size_t)nb * NS);
       std::vector<int> bs(nb);
-      std::vector<int64_t> bc(nb), bg(nb);
+      std::vector<int64_t> bc(nb), bg(nb), bt(nb);
       QPX_HIP(hipMemcpyAsync(bp.data(), bpt2_d.p, sizeof(double) * bp.size(), hipMemcpyDeviceToHost, st));
       QPX_HIP(hipMemcpyAsync(bs.data(), bstat_d.p, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
       QPX_HIP(hipMemcpyAsync(bc.data(), bncand_d.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, st));
       QPX_HIP(hipMemcpyAsync(bg.data(), bngath_d.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, st));
+      QPX_HIP(hipMemcpyAsync(bt.data(), bticks_d.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, st));
       unsigned long long pc = 0;
       QPX_HIP(hipMemcpyAsync(&pc, P.count.p, sizeof(pc), hipMemcpyDeviceToHost, st));
       QPX_HIP(hipStreamSynchronize(st));
@@ -1708,6 +1716,7 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
       bool any_over = false;
       for (int i = 0; i < nb; ++i) {
         int64_t gid = run_gid[i];
+        blk_ticks_all[gid] += bt[i];   // a re-run after a full pool adds its time
         if (bs[i] == 2) {
           any_over = true;
           drop[gid] = 1;
@@ -1764,6 +1773,7 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
     pt2_out[s] = acc;
   }
   if (block_pt2_out) std::memcpy(block_pt2_out, blk_pt2_all.data(), sizeof(double) * blk_pt2_all.size());
+  c->sel_ticks.swap(blk_ticks_all);
   if (block_info_out)
     for (int64_t b = 0; b < n_blocks_total; ++b) {
       const BlockDesc &d = all[b];
@@ -1821,6 +1831,16 @@ extern "C" int qpx_select_blocks(qpx_ctx *c, int n_states, const double *coef, i
                                  int64_t *block_info) {
   return select_impl(c, n_states, coef, n_gen, n_sel, E0, n_keep, 0, pt2, out_dets, out_vals, n_out, keep_mask,
                      block_pt2, block_info);
+}
+
+// benchmark hook: per-block workgroup durations of the last selection call
+// (100 MHz wall-clock ticks, in block enumeration order; 0 for blocks not run)
+extern "C" int qpx_debug_select_block_ticks(qpx_ctx *c, int64_t *ticks, int64_t n) {
+  return qpx_guard(c, [&] {
+    QPX_REQUIRE(ticks, "null output");
+    QPX_REQUIRE(n == (int64_t)c->sel_ticks.size(), "n must equal the block count of the last selection call");
+    std::copy(c->sel_ticks.begin(), c->sel_ticks.end(), ticks);
+  });
 }
 
 // number of blocks of the first n_gen generators (for sizing block arrays)

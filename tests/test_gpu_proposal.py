@@ -40,6 +40,9 @@ def test_learned_proposal():
     r = cipsi(ctx, H, n_det_max=1500, s2_eig=False, log=_say)
     # iteration A: exact labels
     dsA, csA, E0A, pt2A, selA, bA, infoA = _full_sweep(ctx, r.dets, r.coef, r.dets.shape[0])
+    # per-block durations of that sweep (the cost side of a proposal)
+    ticks = ctx._select_block_ticks(infoA.shape[0])
+    assert np.all(ticks >= 0) and np.all(ticks[infoA[:, 5] == 1] > 0)
     desc, fdesc = block_descriptors(H, dsA, csA, dsA.shape[0])
     assert desc.shape[0] == infoA.shape[0]
     spin = np.where(infoA[:, 1] != infoA[:, 3], 2, infoA[:, 1] - 1)
