@@ -67,6 +67,7 @@ def main():
     ap.add_argument("--no-pt2", action="store_true")
     ap.add_argument("--pt2-ndet", type=int, default=100000)
     ap.add_argument("--pt2-seconds", type=float, default=20.0)
+    ap.add_argument("--no-pt2-stoch", action="store_true")
     ap.add_argument("--no-selected", action="store_true")
     ap.add_argument("--sel-small", type=int, default=8000,
                     help="CIPSI loop size before the large selection step of the selected-psi leg")
@@ -548,6 +549,22 @@ def pt2_leg(args, log):
                        "note": "whole qpx_select call (host block enumeration and final sort included)"}
     log(f"bench: pt2 {n_gen} generators: {dt:.2f} s, {st['candidates']} candidates, "
         f"{res['value']:.3e} candidates/s")
+    if not args.no_pt2_stoch:
+        # the full EN-PT2 of this psi (every determinant a generator) is out of
+        # reach exactly; the hybrid stochastic estimate, capped in time
+        from quantum_package_amd.pt2_stoch import ZMQ_pt2
+        t = time.time()
+        est, err, info = ZMQ_pt2(ctx, ds, cs, np.atleast_1d(E0), relative_error=1e-3, absolute_error=1e-9,
+                                 seed=11, combs_per_batch=5, max_seconds=args.pt2_seconds,
+                                 log=lambda m: log("bench: " + m))
+        res["stochastic"] = {"pt2": est.tolist(), "error": err.tolist(), "seconds": time.time() - t,
+                             "combs": info[0]["combs"], "generators_computed": info[0]["generators_computed"],
+                             "generators": info[0]["generators"], "deterministic_set": info[0]["deterministic"],
+                             "workload": "hybrid stochastic EN-PT2 of the same psi, every determinant a generator "
+                                         "and a selector, 100 teeth, 5 combs per batch, stopped at relative "
+                                         f"error 1e-3 or after {args.pt2_seconds:.0f} s"}
+        log(f"bench: stochastic PT2 {est[0]:.8f} +/- {err[0]:.2e} from {info[0]['generators_computed']} of "
+            f"{info[0]['generators']} generators in {time.time() - t:.1f} s")
     if not args.no_cpu:
         # the reference's task farm: (generator, fragment) tasks, fragment
         # count n_alpha^2 (selection.irp.f:561-562), leading generators first
