@@ -206,6 +206,15 @@ def main():
                                          f"FP64 4-vector sigma call, reference algorithm restated in C/OpenMP "
                                          f"schedule(dynamic,64); {dt1:.1f} s"}
         log(f"bench: cpu oracle {rows / dt1:.1f} dets/s on {nthr} threads")
+        # the same restatement on one thread, ~1/3 of the time budget
+        istep1 = max(1, n_det // max(int(rows / dt1 / nthr * args.cpu_seconds / 3), 1))
+        t0 = time.time()
+        rows1, _ = O.sigma_work_sample(ham, bil, nst, istep1, 1)
+        dt2 = time.time() - t0
+        out["cpu_baseline_1thread"] = {"value": rows1 / dt2, "unit": "dets/s", "cores": 1, "kind": "port",
+                                       "sample": f"{rows1} rows (every {istep1}-th), one thread; {dt2:.1f} s"}
+        out["host"] = _host_info(nthr)
+        log(f"bench: cpu oracle {rows1 / dt2:.1f} dets/s on 1 thread")
     del u, v, s
     torch.cuda.empty_cache()
     if not args.no_davidson:
@@ -218,6 +227,22 @@ def main():
     if not args.no_cipsi:
         out["cipsi_step"] = _guarded(cipsi_step_leg, args, log)
     print(json.dumps(out), flush=True)
+
+
+def _host_info(threads_used):
+    """CPU model and logical CPU count of the host the CPU baseline ran on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "cpus_available": aff,
+            "threads_used": threads_used}
 
 
 def _guarded(fn, *a):
