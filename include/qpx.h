@@ -116,6 +116,18 @@ int qpx_select(qpx_ctx *ctx, int n_states, const double *coef, int n_gen, int n_
  * and coefficient gathers. */
 int qpx_select_stats(qpx_ctx *ctx, int64_t *stats4);
 
+/* make_s2_eigenfunction (src/Determinants/occ_pattern.irp.f:251-309): the
+ * determinants that share an occupation pattern with a determinant of
+ * psi_det (N_int,2,n_det) and are not in it, for n_alpha alpha electrons.
+ * Host code, no context.  *n_out receives their count; with out_dets
+ * non-null and max_out >= *n_out they are written there (N_int,2,*n_out)
+ * in a fixed order (patterns grouped by open shells and alpha electrons
+ * among them, then alpha subsets in lexicographic order, then patterns in
+ * word order).  Returns 0, 1 for bad arguments, 2 when max_out is too small
+ * (nothing written, *n_out set). */
+int qpx_make_s2_eigenfunction(int n_int, int64_t n_det, const uint64_t *psi_det, int n_alpha, int64_t max_out,
+                              uint64_t *out_dets, int64_t *n_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,7 +24,8 @@ FLAGS = [
     "-ffp-contract=fast",
 ]
 
-SOURCES = ["context.hip", "wavefunction.hip", "sigma.hip", "sigma_sparse.hip", "davidson.hip", "selection.hip", "mlp.hip"]
+SOURCES = ["context.hip", "wavefunction.hip", "sigma.hip", "sigma_sparse.hip", "davidson.hip", "selection.hip", "mlp.hip",
+           "psi_update.hip"]
 
 
 # the selection kernel keeps the reference's floating-point operation order

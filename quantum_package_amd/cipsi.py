@@ -34,7 +34,6 @@ from __future__ import annotations
 import math
 import time
 from dataclasses import dataclass, field
-from itertools import combinations
 
 import numpy as np
 
@@ -75,63 +74,25 @@ def n_leading(contrib_sorted, threshold, strict=False):
 
 def make_s2_eigenfunction(dets: np.ndarray, n_alpha: int) -> np.ndarray:
     """Determinants sharing an occupation pattern with dets and not in dets
-    (occ_pattern.irp.f:251-309).  dets (N, 2, n_int) uint64."""
-    n_int = dets.shape[2]
-    a = dets[:, 0, :]
-    b = dets[:, 1, :]
-    docc = a & b
-    socc = a ^ b
-    pat = np.concatenate([docc, socc], axis=1)
-    upat = np.unique(pat, axis=0)
-    nd = popcount64(upat[:, :n_int]).sum(axis=1)
-    ns = popcount64(upat[:, n_int:]).sum(axis=1)
-    out = []
-    for n_s in np.unique(ns):
-        sel = ns == n_s
-        P = upat[sel]
-        na_s = n_alpha - nd[sel]
-        for k in np.unique(na_s):
-            Q = P[na_s == k]
-            if n_s == 0 or k < 0 or k > n_s:
-                continue
-            # orbital bit positions of the singly occupied orbitals, ascending
-            pos = np.zeros((Q.shape[0], n_s), dtype=np.int64)
-            cur = np.zeros(Q.shape[0], dtype=np.int64)
-            one = np.uint64(1)
-            for w in range(n_int):
-                x = Q[:, n_int + w].copy()
-                while True:
-                    nz = x != 0
-                    if not nz.any():
-                        break
-                    with np.errstate(over="ignore"):
-                        lb = x & (~x + one)
-                        p = popcount64(lb - one) + 64 * w
-                        rows = np.nonzero(nz)[0]
-                        pos[rows, cur[rows]] = p[rows]
-                        cur[rows] += 1
-                        x = np.where(nz, x & (x - one), x)
-            for comb in combinations(range(n_s), int(k)):
-                alpha = Q[:, :n_int].copy()
-                beta = Q[:, :n_int].copy()
-                for j in range(n_s):
-                    w = pos[:, j] // 64
-                    bitv = np.left_shift(np.uint64(1), (pos[:, j] % 64).astype(np.uint64))
-                    tgt = alpha if j in comb else beta
-                    for ww in range(n_int):
-                        m = w == ww
-                        tgt[m, ww] |= bitv[m]
-                out.append(np.stack([alpha, beta], axis=1))
-    if not out:
-        return np.zeros((0, 2, n_int), dtype=np.uint64)
-    cand = np.concatenate(out, axis=0)
-    ck = det_keys(cand)
-    inpsi = np.isin(ck, det_keys(dets))
-    cand = cand[~inpsi]
-    if cand.shape[0] == 0:
-        return cand
-    _, first = np.unique(det_keys(cand), return_index=True)
-    return cand[np.sort(first)]
+    (occ_pattern.irp.f:251-309), from libqpx's host routine
+    qpx_make_s2_eigenfunction.  dets (N, 2, n_int) uint64."""
+    import ctypes as C
+    from . import _lib
+    L = _lib.load()
+    d = np.ascontiguousarray(dets, dtype=np.uint64)
+    n_int = d.shape[2]
+    n = C.c_int64(0)
+    cap = max(1024, 4 * d.shape[0])
+    while True:
+        out = np.empty((cap, 2, n_int), dtype=np.uint64)
+        rc = L.qpx_make_s2_eigenfunction(n_int, d.shape[0], d.ctypes.data_as(_lib.c_u64p), int(n_alpha), cap,
+                                         out.ctypes.data_as(_lib.c_u64p), C.byref(n))
+        if rc == 2:
+            cap = n.value
+            continue
+        if rc != 0:
+            raise ValueError("qpx_make_s2_eigenfunction: bad arguments")
+        return out[:n.value].copy()
 
 
 @dataclass
