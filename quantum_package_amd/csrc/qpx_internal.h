@@ -130,7 +130,8 @@ struct Wfn {
   DBuf<int> col_ptr, csc_a, csc_b;        // CSC by beta column; entries sorted by alpha
   DBuf<int> row_ptr, csr_b, csr2csc;      // CSR by alpha row; entries sorted by beta
   DBuf<int64_t> perm_d;                   // bilinear position -> psi index
-  SpinConn ca, cb;
+  SpinConn ca, cb;               // built on first use by the dense-map or scan path (ensure_conn)
+  bool conn_valid = false;
   DBuf<double> hdiag, sdiag;              // bilinear order
   // per-string scan costs for the generic alpha/beta kernel
   DBuf<int64_t> cost_a, cost_b;
@@ -174,8 +175,10 @@ namespace qpx {
 // implemented in context.hip: builds Ham::G4/G4f if M <= kG4MaxOrb and
 // returns whether the table exists
 bool ensure_g4(qpx_ctx *c);
-// implemented in wavefunction.hip
+// implemented in wavefunction.hip; ensure_conn builds the spin-string
+// connectivity and scan costs of the current wavefunction if needed
 void build_wavefunction(qpx_ctx *c, int n_int, int64_t n_det, const uint64_t *psi_det);
+void ensure_conn(qpx_ctx *c);
 // implemented in sigma.hip
 void sigma_device(qpx_ctx *c, int prec, int n_st, const void *u_t, void *v_t, void *s_t, hipStream_t s);
 // implemented in sigma_sparse.hip

@@ -681,7 +681,9 @@ static bool dense_ok(qpx_ctx *c, int nst, size_t tsize) {
   const Wfn &W = c->W;
   const Ham &H = c->H;
   if (W.nint > 2 || nst > 4) return false;
-  if (W.n_a > 65534 || H.M > 128 || W.ca.max_singles > 96) return false;
+  if (W.n_a > 65534 || H.M > 128) return false;
+  ensure_conn(c);   // at most 65534 alpha strings here: cheap
+  if (W.ca.max_singles > 96) return false;
   size_t need_c = ((size_t)(W.n_a + 1) * nst + (size_t)H.M * H.M + (size_t)1024 * nst) * tsize;
   size_t need_r = ((size_t)W.n_b * nst + (size_t)H.M * H.M) * tsize;
   size_t lim = (size_t)c->lds_per_block;
@@ -703,9 +705,9 @@ void sigma_device(qpx_ctx *c, int prec, int nst, const void *u, void *v, void *s
   }
   QPX_REQUIRE(c->force_path != 3, "sparse sigma path not applicable to this wavefunction");
   const Wfn &W = c->W;
-  const int maxs = W.ca.max_singles;
   if (dense) {
     c->sigma_path = 1;
+    const int maxs = W.ca.max_singles;
     if (prec == QPX_FP64) {
       if (W.nint == 1) launch_dense_nst<double, 1>(c, nst, maxs, (const double *)u, (double *)v, (double *)s, st);
       else launch_dense_nst<double, 2>(c, nst, maxs, (const double *)u, (double *)v, (double *)s, st);
@@ -715,6 +717,7 @@ void sigma_device(qpx_ctx *c, int prec, int nst, const void *u, void *v, void *s
     }
   } else {
     c->sigma_path = 2;
+    ensure_conn(c);
     if (prec == QPX_FP64) launch_generic<double>(c, nst, (const double *)u, (double *)v, (double *)s, st);
     else launch_generic<float>(c, nst, (const float *)u, (float *)v, (float *)s, st);
   }

@@ -311,16 +311,9 @@ void build_wavefunction(qpx_ctx *c, int nint, int64_t n_det, const uint64_t *psi
   W.csr_b.upload(csr_b, s);
   W.csr2csc.upload(csr2csc, s);
   W.perm_d.upload(W.perm, s);
-  build_conn(c, W.astr, W.n_a, nint, W.ca);
-  build_conn(c, W.bstr, W.n_b, nint, W.cb);
-  // scan costs: alpha string a -> sum over its alpha singles a' of row length
-  W.cost_a.alloc(W.n_a);
-  W.cost_b.alloc(W.n_b);
-  hipLaunchKernelGGL(scan_cost_kernel, dim3((W.n_a + 255) / 256), dim3(256), 0, s, W.ca.s_ptr.p, W.ca.s_idx.p,
-                     W.row_ptr.p, W.n_a, W.cost_a.p);
-  hipLaunchKernelGGL(scan_cost_kernel, dim3((W.n_b + 255) / 256), dim3(256), 0, s, W.cb.s_ptr.p, W.cb.s_idx.p,
-                     W.col_ptr.p, W.n_b, W.cost_b.p);
-  QPX_HIP(hipGetLastError());
+  // the spin-string connectivity (all pairs of unique strings) is built on
+  // first use: the sparse sigma path and the selection do not need it
+  W.conn_valid = false;
   W.hdiag.alloc(n_det);
   W.sdiag.alloc(n_det);
   Ham &H = c->H;
@@ -338,6 +331,25 @@ void build_wavefunction(qpx_ctx *c, int nint, int64_t n_det, const uint64_t *psi
   QPX_HIP(hipGetLastError());
   QPX_HIP(hipStreamSynchronize(s));
   W.valid = true;
+}
+
+void ensure_conn(qpx_ctx *c) {
+  Wfn &W = c->W;
+  QPX_REQUIRE(W.valid, "no wavefunction set");
+  if (W.conn_valid) return;
+  hipStream_t s = c->stream;
+  build_conn(c, W.astr, W.n_a, W.nint, W.ca);
+  build_conn(c, W.bstr, W.n_b, W.nint, W.cb);
+  // scan costs: alpha string a -> sum over its alpha singles a' of row length
+  W.cost_a.alloc(W.n_a);
+  W.cost_b.alloc(W.n_b);
+  hipLaunchKernelGGL(scan_cost_kernel, dim3((W.n_a + 255) / 256), dim3(256), 0, s, W.ca.s_ptr.p, W.ca.s_idx.p,
+                     W.row_ptr.p, W.n_a, W.cost_a.p);
+  hipLaunchKernelGGL(scan_cost_kernel, dim3((W.n_b + 255) / 256), dim3(256), 0, s, W.cb.s_ptr.p, W.cb.s_idx.p,
+                     W.col_ptr.p, W.n_b, W.cost_b.p);
+  QPX_HIP(hipGetLastError());
+  QPX_HIP(hipStreamSynchronize(s));
+  W.conn_valid = true;
 }
 
 }  // namespace qpx
