@@ -116,12 +116,21 @@ __global__ __launch_bounds__(RB) void gram_partial_kernel(const double *__restri
   }
 }
 
-__global__ void reduce_partial_kernel(const double *__restrict__ partial, int nblk, int nout, double *__restrict__ out) {
-  int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= nout) return;
+// out[o] = sum over the nblk partial rows; one workgroup per output, a
+// strided sum per thread and a fixed tree, so the result is deterministic
+__global__ __launch_bounds__(RB) void reduce_partial_kernel(const double *__restrict__ partial, int nblk, int nout,
+                                                            double *__restrict__ out) {
+  __shared__ double red[RB];
+  const int o = blockIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * nout + o];
-  out[o] = s;
+  for (int b = threadIdx.x; b < nblk; b += RB) s += partial[(int64_t)b * nout + o];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = RB / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[o] = red[0];
 }
 
 // X(:, 0:nb) -= A(:, 0:na) C(na x nb)   (C column-major na x nb, in LDS)
@@ -333,8 +342,7 @@ struct Dav {
     hipLaunchKernelGGL(gram_partial_kernel, dim3(nblk), dim3(RB), 0, st, A, lda, na, B, B2, ldb, nb, n, part.p);
     if (B2) nb *= 2;
     int nout = na * nb;
-    hipLaunchKernelGGL(reduce_partial_kernel, dim3((nout + 255) / 256), dim3(256), 0, st, part.p, nblk, nout,
-                       small.p);
+    hipLaunchKernelGGL(reduce_partial_kernel, dim3(nout), dim3(RB), 0, st, part.p, nblk, nout, small.p);
     QPX_HIP(hipGetLastError());
     std::vector<double> tmp(nout);
     QPX_HIP(hipMemcpyAsync(tmp.data(), small.p, sizeof(double) * nout, hipMemcpyDeviceToHost, st));
