@@ -140,11 +140,16 @@ __global__ void sub_gemm_kernel(double *__restrict__ X, int64_t ldx, int nb, con
   for (int x = threadIdx.x; x < na * nb; x += blockDim.x) cs[x] = Cm[x];
   __syncthreads();
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    // the row of A stays in registers: the loops run to the compile-time
+    // bound (na <= 32 is checked at entry) with the unused slots skipped
     double a[32];
-    for (int i = 0; i < na; ++i) a[i] = A[k + lda * i];
+#pragma unroll
+    for (int i = 0; i < 32; ++i) a[i] = (i < na) ? A[k + lda * i] : 0.0;
     for (int j = 0; j < nb; ++j) {
       double s = 0.0;
-      for (int i = 0; i < na; ++i) s += a[i] * cs[i + na * j];
+#pragma unroll
+      for (int i = 0; i < 32; ++i)
+        if (i < na) s += a[i] * cs[i + na * j];
       X[k + ldx * j] -= s;
     }
   }
