@@ -67,6 +67,14 @@ module qpx_mod
       real(c_double) :: u(*), e(*), s2(*)
       integer(c_int) :: nsig(2)
     end function
+    integer(c_int) function qpx_make_s2_eigenfunction(nint, ndet, psi, nalpha, maxout, dets, nout) &
+        bind(C, name="qpx_make_s2_eigenfunction")
+      import :: c_int, c_int64_t
+      integer(c_int), value :: nint, nalpha
+      integer(c_int64_t), value :: ndet, maxout
+      integer(c_int64_t) :: psi(*), dets(*)
+      integer(c_int64_t) :: nout
+    end function
     integer(c_int) function qpx_select(ctx, nstates, coef, ngen, nsel, e0, nkeep, subset, pt2, dets, vals, nout) &
         bind(C, name="qpx_select")
       import :: c_int, c_ptr, c_double, c_int64_t
@@ -147,6 +155,26 @@ contains
     integer(8), intent(out) :: n_sel_out
     call qpx_check(qpx_select(qpx_ctx, int(N_states, c_int), psi_coef_sorted, int(N_det_generators, c_int), &
         int(N_det_selectors, c_int), E0, int(N_in, c_int64_t), 0_c_int, pt2, sel_dets, sel_vals, n_sel_out))
+  end subroutine
+
+  ! make_s2_eigenfunction (occ_pattern.irp.f:251-309): the determinants that
+  ! complete the spin multiplets of psi_det, returned in a buffer allocated
+  ! here; the caller appends them (fill_H_apply_buffer_no_selection)
+  subroutine qpx_s2_completion(psi_det, nint, ndet, n_alpha, new_dets, n_new)
+    integer, intent(in) :: nint, ndet, n_alpha
+    integer(8), intent(in) :: psi_det(nint, 2, ndet)
+    integer(8), allocatable, intent(out) :: new_dets(:,:,:)
+    integer(8), intent(out) :: n_new
+    integer(c_int64_t) :: nout, dummy(1)
+    integer(c_int) :: rc
+    rc = qpx_make_s2_eigenfunction(int(nint, c_int), int(ndet, c_int64_t), psi_det, int(n_alpha, c_int), &
+        0_c_int64_t, dummy, nout)
+    if (rc /= 0 .and. rc /= 2) stop 'qpx_make_s2_eigenfunction: bad arguments'
+    allocate(new_dets(nint, 2, max(nout, 1_c_int64_t)))
+    rc = qpx_make_s2_eigenfunction(int(nint, c_int), int(ndet, c_int64_t), psi_det, int(n_alpha, c_int), &
+        nout, new_dets, nout)
+    if (rc /= 0) stop 'qpx_make_s2_eigenfunction failed'
+    n_new = nout
   end subroutine
 
   subroutine qpx_finalize()

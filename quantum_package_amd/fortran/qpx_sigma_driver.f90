@@ -3,12 +3,15 @@
 ! stream file, runs
 !   call qpx_H_S2_u_0_nstates(v_0,s_0,u_0,N_st,sze)     (H_S2_u_0_nstates_openmp)
 !   call qpx_davidson_diag(...)                          (davidson_diag_hs2)
-! and writes v_0, s_0 and the lowest energy to the output file.
+!   call qpx_s2_completion(...)                          (make_s2_eigenfunction)
+! and writes v_0, s_0, the lowest energy and the determinants completing the
+! spin multiplets of the first third of psi to the output file.
 !
 ! input  (stream, native endianness): int32 mo_num, n_alpha, n_beta, n_int,
 !        int64 n_det, int32 n_st, int64 n8, real64 h(mo_num,mo_num),
 !        real64 eri(n8), int64 psi_det(n_int,2,n_det), real64 u(n_det,n_st)
-! output (stream): real64 v(n_det,n_st), real64 s(n_det,n_st), real64 e(1)
+! output (stream): real64 v(n_det,n_st), real64 s(n_det,n_st), real64 e(1),
+!        int64 n_new, int64 new_dets(n_int,2,n_new)
 program qpx_sigma_driver
   use qpx_mod
   implicit none
@@ -16,8 +19,9 @@ program qpx_sigma_driver
   integer(4) :: mo_num, na, nb, nint, nst
   integer(8) :: ndet, n8
   double precision, allocatable :: h(:,:), eri(:), u(:,:), v(:,:), s(:,:), ud(:,:), e(:), s2(:)
-  integer(8), allocatable :: psi(:,:,:)
-  integer :: un
+  integer(8), allocatable :: psi(:,:,:), newd(:,:,:)
+  integer(8) :: nnew
+  integer :: un, nthird
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
   open(newunit=un, file=trim(fin), access='stream', form='unformatted', status='old')
@@ -32,8 +36,12 @@ program qpx_sigma_driver
   ud = 0.d0
   ud(:,1) = u(:,1)
   call qpx_davidson_diag(ud, s2, int(ndet), e, 1, 4, 1.d-10, 8, .true., .false.)
+  nthird = int(ndet / 3)
+  call qpx_s2_completion(psi(:, :, 1:nthird), int(nint), nthird, int(na), newd, nnew)
   open(newunit=un, file=trim(fout), access='stream', form='unformatted', status='replace')
   write(un) v, s, e(1)
+  write(un) nnew
+  if (nnew > 0) write(un) newd(:, :, 1:nnew)
   close(un)
   call qpx_finalize()
 end program

@@ -42,7 +42,9 @@ def test_fortran_host_sigma_and_davidson(tmp_path):
     _say("[fortran] qpx_sigma_driver on FCI M=8 4a4b")
     r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
-    out = np.fromfile(fout, dtype=np.float64)
+    raw = np.fromfile(fout, dtype=np.uint8)
+    out = raw[: 8 * (4 * n + 1)].view(np.float64)
+    tail = raw[8 * (4 * n + 1):].view(np.int64)
     v = out[: 2 * n].reshape((n, 2), order="F")
     s = out[2 * n: 4 * n].reshape((n, 2), order="F")
     e = out[4 * n]
@@ -53,3 +55,15 @@ def test_fortran_host_sigma_and_davidson(tmp_path):
     assert np.max(np.abs(s - so)) < 1e-11
     eo, _, _, _ = O.davidson(ham, bil, dets, u[:, 0], n_st=1, n_st_diag=4, threshold=1e-10)
     assert abs(e - eo[0]) < 1e-9
+    # S^2 completion of the first third of psi, called from Fortran with
+    # det(N_int,2,N) arrays: the multiplet partners missing from it, each
+    # sharing an occupation pattern with it
+    from quantum_package_amd.cipsi import make_s2_eigenfunction
+    part = dets[: n // 3]
+    want = make_s2_eigenfunction(part, 4)
+    n_new = int(tail[0])
+    got = tail[1:].astype(np.uint64).reshape(n_new, 2, 1)
+    assert n_new == want.shape[0] > 0
+    assert np.array_equal(got, want)
+    pats = {(int(d[0, 0]) & int(d[1, 0]), int(d[0, 0]) ^ int(d[1, 0])) for d in part}
+    assert all((int(d[0, 0]) & int(d[1, 0]), int(d[0, 0]) ^ int(d[1, 0])) in pats for d in got)
