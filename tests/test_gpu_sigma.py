@@ -160,6 +160,44 @@ def test_sigma_sparse_vs_scan_on_a_large_space(ctx):
         assert np.linalg.norm(sf[:, k] - s2[:, k]) < 2e-5 * np.linalg.norm(s2[:, k])
 
 
+@pytest.mark.parametrize("path", [0, 1, 2, 3])
+def test_sigma_single_and_pair(ctx, path):
+    """The smallest wavefunctions: one determinant (v = H_II u, s = S^2_II u)
+    and an open-shell pair of spin partners (2x2 blocks) on every path."""
+    from quantum_package_amd.hamiltonian import list_to_bitstring
+    H = synthetic_hamiltonian(10, 3, 3, seed=40)
+    ham = O.Ham(H)
+    ctx.set_hamiltonian(H)
+    one = hf_determinant(10, 3, 3)[None]
+    a = list_to_bitstring([1, 2, 4], 1)
+    b = list_to_bitstring([1, 2, 5], 1)
+    pair = np.array([[a, b], [b, a]], dtype=np.uint64)
+    _say(f"[sigma edge] one determinant and a spin pair, path={path}")
+    for dets in (one, pair):
+        ctx.set_wavefunction(dets)
+        u = np.random.default_rng(3).standard_normal((dets.shape[0], 2))
+        ctx._force_sigma_path(path)
+        try:
+            v, s = ctx.H_S2_u_0_nstates(u)
+        except Exception as e:
+            ctx._force_sigma_path(0)
+            if path in (1, 3):
+                pytest.skip(f"path {path} not applicable: {e}")
+            raise
+        ctx._force_sigma_path(0)
+        vo, so = O.sigma(ham, O.Bilinear(dets), u)
+        assert np.max(np.abs(v - vo)) < 1e-12 and np.max(np.abs(s - so)) < 1e-12
+    # the pair: <a b|S^2|b a> = -1 (spin exchange), diagonal S^2 = 1
+    assert np.allclose(so, np.array([[1, -1], [-1, 1]]) @ u)
+
+
+def test_empty_wavefunction_is_rejected(ctx):
+    from quantum_package_amd.api import QpxError
+    ctx.set_hamiltonian(synthetic_hamiltonian(8, 4, 4, seed=1))
+    with pytest.raises(QpxError):
+        ctx.set_wavefunction(np.zeros((0, 2, 1), dtype=np.uint64))
+
+
 def test_diag_h(ctx):
     H = synthetic_hamiltonian(70, 3, 3, seed=8)
     dets = _random_space(70, 3, 3, 500, 8)
