@@ -153,11 +153,12 @@ class Context:
         n = n_out.value
         return pt2, out_d[:n].copy(), out_v[:n].copy()
 
-    def select_blocks(self, coef_sorted, n_gen: int, n_sel: int, E0, N: int, keep=None):
+    def select_blocks(self, coef_sorted, n_gen: int, n_sel: int, E0, N: int, keep=None, info: bool = True):
         """Selection restricted to the blocks with keep[b] != 0 (one byte per
         block of the first n_gen generators, in the reference's block order;
         None = all).  Returns (pt2, dets, vals, block_pt2[n_blocks, N_states],
-        block_info[n_blocks, 6] = gen, s1, h1, s2, h2 (1-based), status)."""
+        block_info[n_blocks, 6] = gen, s1, h1, s2, h2 (1-based), status);
+        block_info is None with info=False (it is 48 bytes per block)."""
         coef = np.asfortranarray(coef_sorted, dtype=np.float64)
         if coef.ndim == 1:
             coef = coef[:, None].copy(order="F")
@@ -171,7 +172,7 @@ class Context:
         out_v = np.zeros(max(N, 1))
         n_out = C.c_int64(0)
         bpt2 = np.zeros((nb, ns))
-        binfo = np.zeros((nb, 6), dtype=np.int64)
+        binfo = np.zeros((nb, 6), dtype=np.int64) if info else None
         kp = None
         if keep is not None:
             k = np.ascontiguousarray(keep, dtype=np.uint8)
@@ -180,7 +181,8 @@ class Context:
             kp = k.ctypes.data_as(C.POINTER(C.c_uint8))
         self._check(self.L.qpx_select_blocks(self.ptr, ns, _dp(coef), n_gen, n_sel, _dp(E0), N, kp, _dp(pt2),
                                              out_d.ctypes.data_as(_lib.c_u64p), _dp(out_v), C.byref(n_out),
-                                             _dp(bpt2), binfo.ctypes.data_as(_lib.c_i64p)))
+                                             _dp(bpt2),
+                                             binfo.ctypes.data_as(_lib.c_i64p) if info else None))
         n = n_out.value
         return pt2, out_d[:n].copy(), out_v[:n].copy(), bpt2, binfo
 

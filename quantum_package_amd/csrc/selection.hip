@@ -1623,14 +1623,23 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
     QPX_HIP(hipStreamSynchronize(st));
   };
 
+  // generators with at least one block to run (all of them unless a keep
+  // mask or a subset leaves some out): near lists are built for these only
+  std::vector<int> active;
+  active.reserve(n_gen);
+  for (int g = 1; g <= n_gen; ++g)
+    for (int b = gen_first[g - 1]; b < gen_first[g]; ++b)
+      if (all[b].gslot) {
+        active.push_back(g);
+        break;
+      }
   // batches of generators: bounded near-list size and block count
-  int g0 = 1;
+  size_t a0 = 0;
   int batch_gens = 1;  // the first generator alone establishes tau
-  while (g0 <= n_gen) {
-    int g1 = std::min(n_gen, g0 + batch_gens - 1);
-    int ng = g1 - g0 + 1;
-    std::vector<int> gens(ng);
-    std::iota(gens.begin(), gens.end(), g0);
+  while (a0 < active.size()) {
+    const size_t a1 = std::min(active.size(), a0 + (size_t)batch_gens);
+    const int ng = (int)(a1 - a0);
+    std::vector<int> gens(active.begin() + a0, active.begin() + a1);
     gens_d.upload(gens, st);
     near_cnt_d.alloc(ng);
     near_off_d.alloc(ng);
@@ -1651,14 +1660,16 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
     // blocks of this batch
     std::vector<BlockDesc> bl;
     std::vector<int64_t> bl_gid;
-    for (int g = g0; g <= g1; ++g)
+    for (int q = 0; q < ng; ++q) {
+      const int g = gens[q];
       for (int b = gen_first[g - 1]; b < gen_first[g]; ++b) {
         if (!all[b].gslot) continue;  // filtered out by subset / keep mask
         BlockDesc d = all[b];
-        d.gslot = g - g0;
+        d.gslot = q;
         bl.push_back(d);
         bl_gid.push_back(b);
       }
+    }
     std::vector<int64_t> pending(bl.size());
     std::iota(pending.begin(), pending.end(), 0);
     while (!pending.empty()) {
@@ -1741,10 +1752,11 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
       pending.swap(again);
     }
     if (pool_n > n_keep) compact(false);
-    g0 = g1 + 1;
     // grow the batch: aim at ~ 32k blocks per launch
-    int per_gen = (int)std::max<int64_t>(1, (gen_first[std::min(g1, n_gen)] - gen_first[g1 - 1]));
+    const int glast = gens.back();
+    int per_gen = (int)std::max<int64_t>(1, (gen_first[glast] - gen_first[glast - 1]));
     batch_gens = std::max(1, std::min(4096, 32768 / per_gen));
+    a0 = a1;
   }
   // final: sort the pool by (key, determinant words) and keep n_keep
   std::vector<double> keys(pool_n);

@@ -271,7 +271,7 @@ def ZMQ_pt2(ctx, dets: np.ndarray, coef: np.ndarray, E0, relative_error: float =
             n_gen = int(gens.max())
             keep = np.zeros((n_gen, nbk), dtype=np.uint8)
             keep[gens - 1] = 1
-            _, _, _, bpt2, _ = ctx.select_blocks(cf, n_gen, n_det, E0, 1, keep=keep.ravel())
+            _, _, _, bpt2, _ = ctx.select_blocks(cf, n_gen, n_det, E0, 1, keep=keep.ravel(), info=False)
             return bpt2[:, ist].reshape(n_gen, nbk)[gens - 1].sum(axis=1)
 
         p, e, info = stochastic_pt2_state(cs[:, ist], compute, relative_error, absolute_error, seed=seed + ist,

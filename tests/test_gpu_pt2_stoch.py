@@ -46,3 +46,35 @@ def test_stochastic_pt2_against_the_full_selection():
         assert 0.0 < err[k] < 5e-3 * abs(exact[k])
         assert abs(pt2[k] - exact[k]) < 5 * err[k]
     ctx.close()
+
+
+def test_block_pt2_does_not_depend_on_the_other_blocks():
+    """The stochastic estimate relies on a generator's exact PT2 being the
+    same whatever else is enumerated in the call: block PT2 under a sparse
+    keep mask equals, bit for bit, the same blocks of the full sweep, and a
+    repeated sweep gives the same values."""
+    from quantum_package_amd.api import Context
+    H = synthetic_hamiltonian(24, 5, 5, seed=607)
+    ctx = Context(0)
+    r = cipsi(ctx, H, n_det_max=800, n_states=1, s2_eig=False, log=lambda m: None)
+    ds, cs, _ = sort_by_weight(r.dets, r.coef)
+    n = ds.shape[0]
+    ctx.set_wavefunction(ds)
+    E0 = ctx.u_0_H_u_0(cs[:, 0])
+    cf = np.asfortranarray(cs)
+    _say(f"[pt2_stoch] block PT2 under keep masks, n_det={n}")
+    _, _, _, full, info = ctx.select_blocks(cf, n, n, E0, 1)
+    _, _, _, again, _ = ctx.select_blocks(cf, n, n, E0, 1, info=False)
+    assert np.array_equal(full, again)
+    nbk = full.shape[0] // n
+    rng = np.random.default_rng(5)
+    for trial in range(3):
+        gens = np.sort(rng.choice(np.arange(1, n + 1), size=max(3, n // 20), replace=False))
+        n_gen = int(gens.max())
+        keep = np.zeros((n_gen, nbk), dtype=np.uint8)
+        keep[gens - 1] = 1
+        _, _, _, part, _ = ctx.select_blocks(cf, n_gen, n, E0, 1, keep=keep.ravel(), info=False)
+        sel = keep.ravel().astype(bool)
+        assert np.array_equal(part[sel], full[: n_gen * nbk][sel])
+        assert np.all(part[~sel] == 0.0)
+    ctx.close()
