@@ -116,6 +116,16 @@ int qpx_select(qpx_ctx *ctx, int n_states, const double *coef, int n_gen, int n_
  * and coefficient gathers. */
 int qpx_select_stats(qpx_ctx *ctx, int64_t *stats4);
 
+/* The EN-PT2 contribution of each listed generator, as select_connected
+ * sums it over the generator's blocks (plugins/Full_CI_ZMQ/selection.irp.f:50-79,
+ * the unit of the reference's PT2 tasks, run_pt2_slave.irp.f:2-136).  gens
+ * (n_list) are 1-based indices into psi_det_sorted, in any order, without
+ * repeats, each <= n_sel; the first n_sel determinants are the selectors.
+ * gen_pt2 (n_states, n_list) column-major receives the contributions.  The
+ * wavefunction, coef and E0 are as for qpx_select; nothing is selected. */
+int qpx_select_generators(qpx_ctx *ctx, int n_states, const double *coef, int n_sel, const double *E0, int n_list,
+                          const int *gens, double *gen_pt2);
+
 /* make_s2_eigenfunction (src/Determinants/occ_pattern.irp.f:251-309): the
  * determinants that share an occupation pattern with a determinant of
  * psi_det (N_int,2,n_det) and are not in it, for n_alpha alpha electrons.

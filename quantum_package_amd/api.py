@@ -153,6 +153,20 @@ class Context:
         n = n_out.value
         return pt2, out_d[:n].copy(), out_v[:n].copy()
 
+    def select_generators(self, coef_sorted, n_sel: int, E0, gens) -> np.ndarray:
+        """EN-PT2 contribution of each listed generator (1-based indices into
+        the current psi_det_sorted): (len(gens), N_states)."""
+        coef = np.asfortranarray(coef_sorted, dtype=np.float64)
+        if coef.ndim == 1:
+            coef = coef[:, None].copy(order="F")
+        ns = coef.shape[1]
+        g = np.ascontiguousarray(gens, dtype=np.int32)
+        E0 = np.ascontiguousarray(np.atleast_1d(E0), dtype=np.float64)
+        out = np.zeros((g.size, ns))
+        self._check(self.L.qpx_select_generators(self.ptr, ns, _dp(coef), n_sel, _dp(E0), g.size,
+                                                 g.ctypes.data_as(_lib.c_ip), _dp(out)))
+        return out
+
     def select_blocks(self, coef_sorted, n_gen: int, n_sel: int, E0, N: int, keep=None, info: bool = True):
         """Selection restricted to the blocks with keep[b] != 0 (one byte per
         block of the first n_gen generators, in the reference's block order;

@@ -1484,7 +1484,8 @@ struct Pool {
 template <int NI>
 void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel, const double *E0, int64_t n_keep,
                    int subset, double *pt2_out, uint64_t *out_dets, double *out_vals, int64_t *n_out,
-                   const uint8_t *keep_mask, double *block_pt2_out, int64_t *block_info_out) {
+                   const uint8_t *keep_mask, double *block_pt2_out, int64_t *block_info_out,
+                   const std::vector<int> *gen_list = nullptr, double *gen_pt2_out = nullptr) {
   Wfn &W = c->W;
   Ham &H = c->H;
   hipStream_t st = c->stream;
@@ -1518,8 +1519,14 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
   // all blocks in generator order
   std::vector<BlockDesc> all;
   std::vector<int> gen_first;  // first block of each generator
+  std::vector<uint8_t> listed;
+  if (gen_list) {
+    listed.assign((size_t)n_gen + 1, 0);
+    for (int g : *gen_list) listed[g] = 1;
+  }
   for (int g = 1; g <= n_gen; ++g) {
     gen_first.push_back((int)all.size());
+    if (gen_list && !listed[g]) continue;   // only the listed generators get blocks
     const uint64_t *gd = &psi_h[(size_t)(g - 1) * 2 * NI];
     std::vector<int> hl[2];
     for (int s = 0; s < 2; ++s)
@@ -1785,6 +1792,17 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
     pt2_out[s] = acc;
   }
   if (block_pt2_out) std::memcpy(block_pt2_out, blk_pt2_all.data(), sizeof(double) * blk_pt2_all.size());
+  if (gen_list && gen_pt2_out) {
+    // per generator, its blocks in block order (select_connected's sum)
+    for (size_t i = 0; i < gen_list->size(); ++i) {
+      const int g = (*gen_list)[i];
+      for (int st = 0; st < NS; ++st) {
+        double acc = 0.0;
+        for (int b = gen_first[g - 1]; b < gen_first[g]; ++b) acc += blk_pt2_all[(size_t)b * NS + st];
+        gen_pt2_out[i * NS + st] = acc;
+      }
+    }
+  }
   c->sel_ticks.swap(blk_ticks_all);
   if (block_info_out)
     for (int64_t b = 0; b < n_blocks_total; ++b) {
@@ -1852,6 +1870,37 @@ extern "C" int qpx_debug_select_block_ticks(qpx_ctx *c, int64_t *ticks, int64_t 
     QPX_REQUIRE(ticks, "null output");
     QPX_REQUIRE(n == (int64_t)c->sel_ticks.size(), "n must equal the block count of the last selection call");
     std::copy(c->sel_ticks.begin(), c->sel_ticks.end(), ticks);
+  });
+}
+
+// EN-PT2 of each listed generator (select_connected's sum over its blocks)
+extern "C" int qpx_select_generators(qpx_ctx *c, int n_states, const double *coef, int n_sel, const double *E0,
+                                     int n_list, const int *gens, double *gen_pt2) {
+  return qpx_guard(c, [&] {
+    QPX_HIP(hipSetDevice(c->device));
+    QPX_REQUIRE(c->W.valid, "no wavefunction set");
+    QPX_REQUIRE(n_states >= 1 && n_states <= MAXNS, "n_states must be in 1..4");
+    QPX_REQUIRE(n_list >= 1 && gens && gen_pt2 && coef && E0, "bad arguments");
+    QPX_REQUIRE(n_sel >= 1 && n_sel <= c->W.n_det, "need 1 <= n_sel <= n_det");
+    std::vector<int> list(gens, gens + n_list);
+    int n_gen = 0;
+    {
+      std::vector<int> sorted(list);
+      std::sort(sorted.begin(), sorted.end());
+      QPX_REQUIRE(sorted.front() >= 1 && sorted.back() <= n_sel, "generator indices must be in 1..n_sel");
+      QPX_REQUIRE(std::adjacent_find(sorted.begin(), sorted.end()) == sorted.end(), "repeated generator index");
+      n_gen = sorted.back();
+    }
+    double pt2[MAXNS];
+    uint64_t dummy_det[2 * NINT_MAX];
+    double dummy_val;
+    int64_t n_out = 0;
+    switch (c->W.nint) {
+      case 1: run_selection<1>(c, n_states, coef, n_gen, n_sel, E0, 1, 0, pt2, dummy_det, &dummy_val, &n_out, nullptr, nullptr, nullptr, &list, gen_pt2); break;
+      case 2: run_selection<2>(c, n_states, coef, n_gen, n_sel, E0, 1, 0, pt2, dummy_det, &dummy_val, &n_out, nullptr, nullptr, nullptr, &list, gen_pt2); break;
+      case 3: run_selection<3>(c, n_states, coef, n_gen, n_sel, E0, 1, 0, pt2, dummy_det, &dummy_val, &n_out, nullptr, nullptr, nullptr, &list, gen_pt2); break;
+      default: run_selection<4>(c, n_states, coef, n_gen, n_sel, E0, 1, 0, pt2, dummy_det, &dummy_val, &n_out, nullptr, nullptr, nullptr, &list, gen_pt2); break;
+    }
   });
 }
 

@@ -23,8 +23,8 @@ state, with generators sorted by that state's weight c_I^2:
   |error / estimate| < relative_error or |error| < absolute_error.
 
 The exact PT2 of a set of generators is what the selection computes for
-them (``select_connected``): here one ``qpx_select_blocks`` call on the GPU
-restricted to those generators' blocks, summed per generator.
+them (``select_connected``): here one ``qpx_select_generators`` call on the
+GPU for the batch, which runs those generators' blocks only.
 
 Differences from the reference, on purpose:
 
@@ -258,8 +258,6 @@ def ZMQ_pt2(ctx, dets: np.ndarray, coef: np.ndarray, E0, relative_error: float =
         ctx.set_wavefunction(ds)
         pt2, _, _ = ctx.ZMQ_selection(np.asfortranarray(cs), n_det, n_det, E0, 1)
         return pt2, err, [{"exact": True}] * ns
-    na, nb = ctx.H.n_alpha, ctx.H.n_beta
-    nbk = na * (na - 1) // 2 + nb * (nb - 1) // 2 + na * nb
     for ist in range(ns):
         w = np.zeros(ns)
         w[ist] = 1.0
@@ -268,11 +266,7 @@ def ZMQ_pt2(ctx, dets: np.ndarray, coef: np.ndarray, E0, relative_error: float =
         cf = np.asfortranarray(cs)
 
         def compute(gens, cf=cf, ist=ist):
-            n_gen = int(gens.max())
-            keep = np.zeros((n_gen, nbk), dtype=np.uint8)
-            keep[gens - 1] = 1
-            _, _, _, bpt2, _ = ctx.select_blocks(cf, n_gen, n_det, E0, 1, keep=keep.ravel(), info=False)
-            return bpt2[:, ist].reshape(n_gen, nbk)[gens - 1].sum(axis=1)
+            return ctx.select_generators(cf, n_det, E0, gens)[:, ist]
 
         p, e, info = stochastic_pt2_state(cs[:, ist], compute, relative_error, absolute_error, seed=seed + ist,
                                           combs_per_batch=combs_per_batch, log=log, max_seconds=max_seconds)

@@ -77,4 +77,12 @@ def test_block_pt2_does_not_depend_on_the_other_blocks():
         sel = keep.ravel().astype(bool)
         assert np.array_equal(part[sel], full[: n_gen * nbk][sel])
         assert np.all(part[~sel] == 0.0)
+        # per-generator entry point: each generator's blocks summed in block order
+        order = rng.permutation(gens)
+        got = ctx.select_generators(cf, n, E0, order)[:, 0]
+        for g, v in zip(order, got):
+            acc = 0.0
+            for x in full[(g - 1) * nbk: g * nbk, 0]:
+                acc += float(x)
+            assert v == acc
     ctx.close()
