@@ -67,6 +67,14 @@ module qpx_mod
       real(c_double) :: u(*), e(*), s2(*)
       integer(c_int) :: nsig(2)
     end function
+    integer(c_int) function qpx_select_generators(ctx, nstates, coef, nsel, e0, nlist, gens, genpt2) &
+        bind(C, name="qpx_select_generators")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nstates, nsel, nlist
+      real(c_double) :: coef(*), e0(*), genpt2(*)
+      integer(c_int) :: gens(*)
+    end function
     integer(c_int) function qpx_make_s2_eigenfunction(nint, ndet, psi, nalpha, maxout, dets, nout) &
         bind(C, name="qpx_make_s2_eigenfunction")
       import :: c_int, c_int64_t
@@ -155,6 +163,20 @@ contains
     integer(8), intent(out) :: n_sel_out
     call qpx_check(qpx_select(qpx_ctx, int(N_states, c_int), psi_coef_sorted, int(N_det_generators, c_int), &
         int(N_det_selectors, c_int), E0, int(N_in, c_int64_t), 0_c_int, pt2, sel_dets, sel_vals, n_sel_out))
+  end subroutine
+
+  ! pt2_detail(:, i) for the generators gens(i), as the reference's PT2 task
+  ! (run_pt2_slave.irp.f) gets it from select_connected, on psi_det_sorted
+  subroutine qpx_pt2_generators(psi_coef_sorted, ndet, N_states, N_det_selectors, E0, n_list, gens, pt2_detail)
+    integer, intent(in) :: ndet, N_states, N_det_selectors, n_list
+    double precision, intent(in) :: psi_coef_sorted(ndet, N_states), E0(N_states)
+    integer, intent(in) :: gens(n_list)
+    double precision, intent(out) :: pt2_detail(N_states, n_list)
+    integer(c_int), allocatable :: g(:)
+    allocate(g(n_list))
+    g = int(gens, c_int)
+    call qpx_check(qpx_select_generators(qpx_ctx, int(N_states, c_int), psi_coef_sorted, &
+        int(N_det_selectors, c_int), E0, int(n_list, c_int), g, pt2_detail))
   end subroutine
 
   ! make_s2_eigenfunction (occ_pattern.irp.f:251-309): the determinants that

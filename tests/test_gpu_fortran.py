@@ -62,8 +62,23 @@ def test_fortran_host_sigma_and_davidson(tmp_path):
     part = dets[: n // 3]
     want = make_s2_eigenfunction(part, 4)
     n_new = int(tail[0])
-    got = tail[1:].astype(np.uint64).reshape(n_new, 2, 1)
+    got = tail[1: 1 + 2 * n_new].astype(np.uint64).reshape(n_new, 2, 1)
+    pt2_f = tail[1 + 2 * n_new:].view(np.float64)
     assert n_new == want.shape[0] > 0
     assert np.array_equal(got, want)
     pats = {(int(d[0, 0]) & int(d[1, 0]), int(d[0, 0]) ^ int(d[1, 0])) for d in part}
     assert all((int(d[0, 0]) & int(d[1, 0]), int(d[0, 0]) ^ int(d[1, 0])) in pats for d in got)
+    # per-generator PT2 of that third (u(:,1) as coefficients) through the
+    # Fortran wrapper, against the Python entry point
+    from quantum_package_amd.api import Context
+    ctx = Context(0)
+    ctx.set_hamiltonian(H)
+    ctx.set_wavefunction(part)
+    c = u[: n // 3, :1]
+    hc, _ = ctx.H_S2_u_0_nstates(c)
+    e0 = float(c[:, 0] @ hc[:, 0] / (c[:, 0] @ c[:, 0]))
+    want_pt2 = ctx.select_generators(c, n // 3, np.array([e0]), [1, 2, 3, 4])[:, 0]
+    ctx.close()
+    assert pt2_f.shape == (4,)
+    np.testing.assert_allclose(pt2_f, want_pt2, rtol=1e-12, atol=1e-15)
+    assert np.any(pt2_f != 0.0)
