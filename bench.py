@@ -498,6 +498,10 @@ def cipsi_step_leg(args, log):
         log(f"bench: davidson {name} on {dets.shape[0]} dets: {res[name][0]:.2f}s, E = {e[:ns]}, calls {nsig}")
     t_plain = res["fp64"][0] + t_full
     t_learned = res["mixed"][0] + t_prop + t_pred
+    # the network is trained on each full sweep (every second iteration):
+    # charged here to the proposed iteration that follows it
+    trains = [h.get("t_train") for h in r.history if isinstance(h, dict) and h.get("t_train")]
+    t_train = float(trains[-1]) if trains else 0.0
     mlp.close()
     ctx.close()
     return {"workload": f"one CIPSI step, synthetic Hamiltonian seed {seed}: {M} MOs, {na} alpha + {nb} beta "
@@ -506,6 +510,8 @@ def cipsi_step_leg(args, log):
                         f"N_det/2+1 = {to_select} with {n_gen} generators and {n_sel} selectors, then Davidson "
                         f"(threshold 1e-10) on the {dets.shape[0]} determinants of the grown psi",
             "plain_seconds": t_plain, "learned_seconds": t_learned, "speedup": t_plain / t_learned,
+            "training_seconds_last_sweep": t_train,
+            "speedup_with_training": t_plain / (t_learned + t_train),
             "selection_full_seconds": t_full, "selection_proposed_seconds": t_prop,
             "proposal_overhead_seconds": t_pred, "blocks_kept": float(keep.mean()), "missed_pt2_fraction": missed,
             "kept_kernel_time_fraction": float(ticks[keep_b].sum() / ticks.sum()),

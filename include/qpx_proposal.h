@@ -50,6 +50,15 @@ int qpx_mlp_train_step(qpx_mlp *mlp, int n, const int *desc, const float *fdesc,
                        float *loss);
 int qpx_mlp_predict(qpx_mlp *mlp, int n, const int *desc, const float *fdesc, float *out);
 
+/* `epochs` epochs of Adam steps over a training set of n samples (same
+ * layouts as qpx_mlp_train_step) uploaded once: each epoch orders the
+ * samples by a permutation drawn from a SplitMix64 stream seeded with seed
+ * and takes consecutive batches of `batch` (a last partial batch is
+ * dropped; 1 <= batch <= n), gathered on the device.  epoch_loss (epochs)
+ * receives each epoch's mean batch loss. */
+int qpx_mlp_train_epochs(qpx_mlp *mlp, int n, const int *desc, const float *fdesc, const float *target, float lr,
+                         int epochs, int batch, uint64_t seed, float *epoch_loss);
+
 /* Read (set = 0) or write (set = 1) a parameter tensor or, with grad = 1,
  * its last gradient: which = 0 W1, 1 W2, 2 W3, 3 b1, 4 b2, 5 b3, 6 w4,
  * 7 embedding, 8 b4; n elements, row-major. */

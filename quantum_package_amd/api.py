@@ -281,6 +281,17 @@ class BlockMLP:
                                               C.byref(loss)))
         return loss.value
 
+    def train_epochs(self, desc, fdesc, target, lr, epochs: int, batch: int, seed: int = 0):
+        """epochs of Adam steps on batches of a device-resident training set;
+        returns the mean batch loss of each epoch."""
+        d, f = self._prep(desc, fdesc)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        losses = np.zeros(epochs, dtype=np.float32)
+        self._check(self.L.qpx_mlp_train_epochs(self.ptr, d.shape[0], d.ctypes.data_as(_lib.c_ip),
+                                                f.ctypes.data_as(_c_fp), t.ctypes.data_as(_c_fp), lr, epochs,
+                                                batch, seed, losses.ctypes.data_as(_c_fp)))
+        return losses
+
     def predict(self, desc, fdesc):
         d, f = self._prep(desc, fdesc)
         out = np.zeros(d.shape[0], dtype=np.float32)

@@ -17,6 +17,7 @@
 // wavefronts each owning 32x32 = 2x2 tiles of v_mfma_f32_16x16x16_bf16.
 // Lane layout of the 16x16x16 MFMA: lane l holds A[l%16][4(l/16)+0..3],
 // B[4(l/16)+0..3][l%16] and D[4(l/16)+0..3][l%16].
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -193,14 +194,29 @@ __global__ void relu_mask_kernel(float *__restrict__ dH, const float *__restrict
 
 // column sums of a (n x m) row-major fp32 matrix (bias gradients), and the
 // head weight gradient dw4[k] = sum_i H[i,k] dy[i]
-__global__ void colsum_kernel(const float *__restrict__ A, const float *__restrict__ w, int n, int m,
-                              float *__restrict__ out) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m) return;
+// out[k] = sum_i A[i][k] (w[i]): a workgroup of 256 threads per CB columns
+// (CB = 32, or m when m < 32); CB consecutive threads read one row's
+// columns (coalesced), 256 / CB row groups stride the rows, and the groups
+// are summed in order (deterministic).  Launch with ceil(m / CB) groups.
+__global__ __launch_bounds__(256) void colsum_kernel(const float *__restrict__ A, const float *__restrict__ w, int n,
+                                                     int m, float *__restrict__ out) {
+  __shared__ float red[256];
+  const int cb = m < 32 ? m : 32, groups = 256 / cb;
+  const int t = threadIdx.x, c = t % cb, g = t / cb;
+  const int k = blockIdx.x * cb + c;
   float s = 0.f;
-  for (int i = 0; i < n; ++i) s += A[(int64_t)i * m + k] * (w ? w[i] : 1.f);
-  out[k] = s;
+  if (g < groups && k < m)
+    for (int i = g; i < n; i += groups) s += A[(int64_t)i * m + k] * (w ? w[i] : 1.f);
+  red[t] = s;
+  __syncthreads();
+  if (t < cb && k < m) {
+    float acc = 0.f;
+    for (int q = 0; q < groups; ++q) acc += red[q * cb + t];
+    out[k] = acc;
+  }
 }
+
+static inline int colsum_grid(int m) { return m < 32 ? 1 : (m + 31) / 32; }
 
 __global__ void emb_grad_kernel(const int *__restrict__ desc, const float *__restrict__ dX, int n, int din,
                                 float *__restrict__ demb) {
@@ -295,27 +311,28 @@ float train_step(qpx_mlp *m, int n, float lr) {
   hipLaunchKernelGGL(head_kernel, dim3(nb), dim3(256), 0, st, m->Z3.p, m->w4.p, m->b4, n, Hd, m->tgt.p, m->y.p,
                      m->dy.p, m->lossp.p);
   // head grads
-  hipLaunchKernelGGL(colsum_kernel, dim3((Hd + 255) / 256), dim3(256), 0, st, m->Z3.p, m->dy.p, n, Hd, m->gw4.p);
-  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(256), 0, st, m->dy.p, (const float *)nullptr, n, 1, m->gb4.p);
+  hipLaunchKernelGGL(colsum_kernel, dim3(colsum_grid(Hd)), dim3(256), 0, st, m->Z3.p, m->dy.p, n, Hd, m->gw4.p);
+  hipLaunchKernelGGL(colsum_kernel, dim3(colsum_grid(1)), dim3(256), 0, st, m->dy.p, (const float *)nullptr, n, 1,
+                     m->gb4.p);
   hipLaunchKernelGGL(head_back_kernel, dim3(grid1((int64_t)n * Hd)), dim3(256), 0, st, m->Z3.p, m->w4.p, m->dy.p,
                      n, Hd, m->dZ3.p, m->dZ3b.p);
   // layer 3: dW3 = H2^T dZ3 (H x H, K = n); db3 = colsum dZ3; dH2 = dZ3 W3^T
   gemm(st, m->H2b.p, 1, Hd, m->dZ3b.p, Hd, 1, m->gW3.p, Hd, Hd, Hd, n, nullptr, 0, nullptr, 0.f);
-  hipLaunchKernelGGL(colsum_kernel, dim3((Hd + 255) / 256), dim3(256), 0, st, m->dZ3.p, (const float *)nullptr, n,
+  hipLaunchKernelGGL(colsum_kernel, dim3(colsum_grid(Hd)), dim3(256), 0, st, m->dZ3.p, (const float *)nullptr, n,
                      Hd, m->gb3.p);
   gemm(st, m->dZ3b.p, Hd, 1, m->W3b.p, 1, Hd, m->dZ2.p, Hd, n, Hd, Hd, nullptr, 0, nullptr, 0.f);
   hipLaunchKernelGGL(relu_mask_kernel, dim3(grid1((int64_t)n * Hd)), dim3(256), 0, st, m->dZ2.p, m->Z2.p,
                      (int64_t)n * Hd, m->dZ2b.p);
   // layer 2
   gemm(st, m->H1b.p, 1, Hd, m->dZ2b.p, Hd, 1, m->gW2.p, Hd, Hd, Hd, n, nullptr, 0, nullptr, 0.f);
-  hipLaunchKernelGGL(colsum_kernel, dim3((Hd + 255) / 256), dim3(256), 0, st, m->dZ2.p, (const float *)nullptr, n,
+  hipLaunchKernelGGL(colsum_kernel, dim3(colsum_grid(Hd)), dim3(256), 0, st, m->dZ2.p, (const float *)nullptr, n,
                      Hd, m->gb2.p);
   gemm(st, m->dZ2b.p, Hd, 1, m->W2b.p, 1, Hd, m->dZ1.p, Hd, n, Hd, Hd, nullptr, 0, nullptr, 0.f);
   hipLaunchKernelGGL(relu_mask_kernel, dim3(grid1((int64_t)n * Hd)), dim3(256), 0, st, m->dZ1.p, m->Z1.p,
                      (int64_t)n * Hd, m->dZ1b.p);
   // layer 1: dW1 = X^T dZ1 (D x H); dX = dZ1 W1^T (n x D)
   gemm(st, m->Xb.p, 1, D, m->dZ1b.p, Hd, 1, m->gW1.p, Hd, D, Hd, n, nullptr, 0, nullptr, 0.f);
-  hipLaunchKernelGGL(colsum_kernel, dim3((Hd + 255) / 256), dim3(256), 0, st, m->dZ1.p, (const float *)nullptr, n,
+  hipLaunchKernelGGL(colsum_kernel, dim3(colsum_grid(Hd)), dim3(256), 0, st, m->dZ1.p, (const float *)nullptr, n,
                      Hd, m->gb1.p);
   gemm(st, m->dZ1b.p, Hd, 1, m->W1b.p, 1, Hd, m->dX.p, D, n, D, Hd, nullptr, 0, nullptr, 0.f);
   QPX_HIP(hipMemsetAsync(m->gemb.p, 0, sizeof(float) * m->emb.n, st));
@@ -352,6 +369,20 @@ float train_step(qpx_mlp *m, int n, float lr) {
   double l = 0.0;
   for (float x : lp) l += x;
   return (float)(l / n);
+}
+
+// batch rows of the device-resident training set, in permutation order
+__global__ void gather_batch_kernel(const int *__restrict__ perm, int nb, const int *__restrict__ desc_all,
+                                    const float *__restrict__ fdesc_all, const float *__restrict__ tgt_all,
+                                    int *__restrict__ desc, float *__restrict__ fdesc, float *__restrict__ tgt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const int64_t r = perm[i];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) desc[4 * (int64_t)i + q] = desc_all[4 * r + q];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) fdesc[3 * (int64_t)i + q] = fdesc_all[3 * r + q];
+  tgt[i] = tgt_all[r];
 }
 
 template <typename F>
@@ -468,6 +499,42 @@ extern "C" int qpx_mlp_train_step(qpx_mlp *m, int n, const int *desc, const floa
     QPX_HIP(hipMemcpyAsync(m->tgt.p, target, sizeof(float) * n, hipMemcpyHostToDevice, m->st));
     float l = train_step(m, n, lr);
     if (loss) *loss = l;
+  });
+}
+
+extern "C" int qpx_mlp_train_epochs(qpx_mlp *m, int n, const int *desc, const float *fdesc, const float *target,
+                                    float lr, int epochs, int batch, uint64_t seed, float *epoch_loss) {
+  return mguard(m, [&] {
+    QPX_HIP(hipSetDevice(m->device));
+    QPX_REQUIRE(n > 0 && batch >= 1 && batch <= n && epochs >= 1, "need 1 <= batch <= n and epochs >= 1");
+    QPX_REQUIRE(desc && fdesc && target && epoch_loss, "null argument");
+    ensure(m, batch);
+    DBuf<int> desc_all, perm_d;
+    DBuf<float> fdesc_all, tgt_all;
+    desc_all.upload(desc, (size_t)4 * n, m->st);
+    fdesc_all.upload(fdesc, (size_t)3 * n, m->st);
+    tgt_all.upload(target, (size_t)n, m->st);
+    perm_d.alloc(n);
+    std::vector<int> perm(n);
+    uint64_t s = seed;
+    for (int ep = 0; ep < epochs; ++ep) {
+      for (int i = 0; i < n; ++i) perm[i] = i;
+      for (int i = n - 1; i > 0; --i) {   // Fisher-Yates
+        const int j = (int)(sm64(s) % (uint64_t)(i + 1));
+        std::swap(perm[i], perm[j]);
+      }
+      QPX_HIP(hipMemcpyAsync(perm_d.p, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, m->st));
+      double lsum = 0.0;
+      int steps = 0;
+      for (int b0 = 0; b0 + batch <= n; b0 += batch) {
+        hipLaunchKernelGGL(gather_batch_kernel, dim3((batch + 255) / 256), dim3(256), 0, m->st, perm_d.p + b0, batch,
+                           desc_all.p, fdesc_all.p, tgt_all.p, m->desc.p, m->fdesc.p, m->tgt.p);
+        QPX_HIP(hipGetLastError());
+        lsum += train_step(m, batch, lr);
+        ++steps;
+      }
+      epoch_loss[ep] = (float)(lsum / std::max(steps, 1));
+    }
   });
 }
 

@@ -126,14 +126,19 @@ class Trainer:
         t = ((y - self.mu) / self.sd).astype(np.float32)
         f = self._norm_f(fdesc)
         n = desc.shape[0]
+        if n >= self.batch:
+            # the whole training set on the device, batches gathered there
+            seed = int(self.rng.integers(1 << 62))
+            losses = self.mlp.train_epochs(desc, f, t, self.lr, epochs, self.batch, seed)
+            if log:
+                for ep, l in enumerate(losses):
+                    log(f"mlp epoch {ep}: loss {l:.4f}")
+            return list(losses)
         losses = []
         for ep in range(epochs):
-            perm = self.rng.permutation(n)
-            for b0 in range(0, n - self.batch + 1 if n >= self.batch else 1, self.batch):
-                idx = perm[b0:b0 + self.batch]
-                losses.append(self.mlp.train_step(desc[idx], f[idx], t[idx], self.lr))
+            losses.append(self.mlp.train_step(desc, f, t, self.lr))
             if log:
-                log(f"mlp epoch {ep}: loss {np.mean(losses[-50:]):.4f}")
+                log(f"mlp epoch {ep}: loss {losses[-1]:.4f}")
         return losses
 
     def predict(self, desc, fdesc):
