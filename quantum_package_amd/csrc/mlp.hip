@@ -51,6 +51,8 @@ struct GemmArgs {
   int relu;           // epilogue: C = relu(C + bias)
   uint16_t *Cb;       // optional bf16 copy of C (same layout)
   float beta;         // C = acc + beta * C_old (gradient accumulation)
+  float *part;        // split-K partial products (ksplit x M x N), or null
+  int ksplit;         // K is cut into ksplit chunks (blockIdx.z); 1 = no split
 };
 
 constexpr int TM = 64, TN = 64, TK = 16;
@@ -66,18 +68,22 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < g.K; k0 += TK) {
+  // this workgroup's slice of K (all of it without a split); slices are
+  // whole multiples of TK
+  const int kc = ((g.K + g.ksplit - 1) / g.ksplit + TK - 1) / TK * TK;
+  const int k_lo = (int)blockIdx.z * kc, k_hi = min(g.K, k_lo + kc);
+  for (int k0 = k_lo; k0 < k_hi; k0 += TK) {
     __syncthreads();
     // stage A tile (TM x TK) and B tile (TK x TN): 1024 elements each
     for (int x = tid; x < TM * TK; x += 256) {
       int ii = x / TK, kk = x - ii * TK;
       int gi = i0 + ii, gk = k0 + kk;
-      As[kk][ii] = (gi < g.M && gk < g.K) ? g.A[gi * g.a_rs + gk * g.a_cs] : (uint16_t)0;
+      As[kk][ii] = (gi < g.M && gk < k_hi) ? g.A[gi * g.a_rs + gk * g.a_cs] : (uint16_t)0;
     }
     for (int x = tid; x < TK * TN; x += 256) {
       int kk = x / TN, jj = x - kk * TN;
       int gk = k0 + kk, gj = j0 + jj;
-      Bs[kk][jj] = (gk < g.K && gj < g.N) ? g.B[gk * g.b_rs + gj * g.b_cs] : (uint16_t)0;
+      Bs[kk][jj] = (gk < k_hi && gj < g.N) ? g.B[gk * g.b_rs + gj * g.b_cs] : (uint16_t)0;
     }
     __syncthreads();
     const int r = lane & 15, q = (lane >> 4) * 4;
@@ -108,7 +114,9 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         int gi = i0 + wi + 16 * a + q + t, gj = j0 + wj + 16 * b + r;
-        if (gi < g.M && gj < g.N) {
+        if (gi < g.M && gj < g.N && g.ksplit > 1) {
+          g.part[((int64_t)blockIdx.z * g.M + gi) * g.N + gj] = acc[a][b][t];
+        } else if (gi < g.M && gj < g.N) {
           float v = acc[a][b][t];
           float *c = g.C + gi * g.c_rs + gj;
           if (g.beta != 0.f) v += g.beta * (*c);
@@ -118,6 +126,22 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
           if (g.Cb) g.Cb[gi * g.c_rs + gj] = f2bf(v);
         }
       }
+}
+
+// split-K: C = sum of the partial products in slice order, then the epilogue
+__global__ void gemm_splitk_reduce_kernel(GemmArgs g) {
+  const int64_t mn = (int64_t)g.M * g.N;
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < mn; x += (int64_t)gridDim.x * blockDim.x) {
+    const int gi = (int)(x / g.N), gj = (int)(x - (int64_t)gi * g.N);
+    float v = 0.f;
+    for (int z = 0; z < g.ksplit; ++z) v += g.part[(int64_t)z * mn + x];
+    float *c = g.C + gi * g.c_rs + gj;
+    if (g.beta != 0.f) v += g.beta * (*c);
+    if (g.bias) v += g.bias[gj];
+    if (g.relu) v = v > 0.f ? v : 0.f;
+    *c = v;
+    if (g.Cb) g.Cb[gi * g.c_rs + gj] = f2bf(v);
+  }
 }
 
 __global__ void to_bf16_kernel(const float *__restrict__ x, uint16_t *__restrict__ y, int64_t n) {
@@ -255,6 +279,7 @@ struct qpx_mlp {
   DBuf<float> W1, W2, W3, b1, b2, b3, w4, emb;
   DBuf<uint16_t> W1b, W2b, W3b;
   DBuf<float> gW1, gW2, gW3, gb1, gb2, gb3, gw4, gemb, gb4;
+  DBuf<float> ws;   // split-K partial products of the weight gradients (sized once)
   DBuf<float> mW1, mW2, mW3, mb1, mb2, mb3, mw4, memb, vW1, vW2, vW3, vb1, vb2, vb3, vw4, vemb;
   float b4 = 0.f, mb4 = 0.f, vb4 = 0.f;
   int step = 0;
@@ -269,12 +294,32 @@ struct qpx_mlp {
 
 namespace {
 
+// C = A B (+ epilogue).  With a workspace, a product whose tile grid
+// leaves most CUs idle and whose K is long (the weight gradients, K = batch)
+// is cut along K across workgroups and reduced in slice order.
 void gemm(hipStream_t st, const uint16_t *A, int64_t ars, int64_t acs, const uint16_t *B, int64_t brs, int64_t bcs,
-          float *C, int64_t crs, int M, int N, int K, const float *bias, int relu, uint16_t *Cb, float beta) {
-  GemmArgs g{A, ars, acs, B, brs, bcs, C, crs, M, N, K, bias, relu, Cb, beta};
+          float *C, int64_t crs, int M, int N, int K, const float *bias, int relu, uint16_t *Cb, float beta,
+          DBuf<float> *ws = nullptr) {
+  GemmArgs g{A, ars, acs, B, brs, bcs, C, crs, M, N, K, bias, relu, Cb, beta, nullptr, 1};
   dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
+  const int tiles = (int)(grid.x * grid.y);
+  if (ws && tiles < 128 && K >= 8 * TK) {
+    g.ksplit = std::max(1, std::min({256 / tiles, K / (4 * TK), 32}));
+    if (g.ksplit > 1 && ws->n >= (size_t)g.ksplit * M * N) {
+      g.part = ws->p;
+      grid.z = g.ksplit;
+    } else {
+      g.ksplit = 1;
+    }
+  }
   hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, st, g);
   QPX_HIP(hipGetLastError());
+  if (g.ksplit > 1) {
+    const int64_t mn = (int64_t)M * N;
+    hipLaunchKernelGGL(gemm_splitk_reduce_kernel, dim3((unsigned)std::min<int64_t>((mn + 255) / 256, 4096)), dim3(256),
+                       0, st, g);
+    QPX_HIP(hipGetLastError());
+  }
 }
 
 void ensure(qpx_mlp *m, int n) {
@@ -287,6 +332,7 @@ void ensure(qpx_mlp *m, int n) {
   m->dZ1b.alloc((size_t)c * m->hid); m->dZ2b.alloc((size_t)c * m->hid); m->dZ3b.alloc((size_t)c * m->hid);
   m->y.alloc(c); m->dy.alloc(c); m->lossp.alloc(1024);
   m->desc.alloc((size_t)4 * c); m->fdesc.alloc((size_t)3 * c); m->tgt.alloc(c);
+  if (!m->ws.p) m->ws.alloc((size_t)32 * m->hid * m->hid);   // the largest split: 32 x hid x hid
   m->cap = c;
 }
 
@@ -317,21 +363,21 @@ float train_step(qpx_mlp *m, int n, float lr) {
   hipLaunchKernelGGL(head_back_kernel, dim3(grid1((int64_t)n * Hd)), dim3(256), 0, st, m->Z3.p, m->w4.p, m->dy.p,
                      n, Hd, m->dZ3.p, m->dZ3b.p);
   // layer 3: dW3 = H2^T dZ3 (H x H, K = n); db3 = colsum dZ3; dH2 = dZ3 W3^T
-  gemm(st, m->H2b.p, 1, Hd, m->dZ3b.p, Hd, 1, m->gW3.p, Hd, Hd, Hd, n, nullptr, 0, nullptr, 0.f);
+  gemm(st, m->H2b.p, 1, Hd, m->dZ3b.p, Hd, 1, m->gW3.p, Hd, Hd, Hd, n, nullptr, 0, nullptr, 0.f, &m->ws);
   hipLaunchKernelGGL(colsum_kernel, dim3(colsum_grid(Hd)), dim3(256), 0, st, m->dZ3.p, (const float *)nullptr, n,
                      Hd, m->gb3.p);
   gemm(st, m->dZ3b.p, Hd, 1, m->W3b.p, 1, Hd, m->dZ2.p, Hd, n, Hd, Hd, nullptr, 0, nullptr, 0.f);
   hipLaunchKernelGGL(relu_mask_kernel, dim3(grid1((int64_t)n * Hd)), dim3(256), 0, st, m->dZ2.p, m->Z2.p,
                      (int64_t)n * Hd, m->dZ2b.p);
   // layer 2
-  gemm(st, m->H1b.p, 1, Hd, m->dZ2b.p, Hd, 1, m->gW2.p, Hd, Hd, Hd, n, nullptr, 0, nullptr, 0.f);
+  gemm(st, m->H1b.p, 1, Hd, m->dZ2b.p, Hd, 1, m->gW2.p, Hd, Hd, Hd, n, nullptr, 0, nullptr, 0.f, &m->ws);
   hipLaunchKernelGGL(colsum_kernel, dim3(colsum_grid(Hd)), dim3(256), 0, st, m->dZ2.p, (const float *)nullptr, n,
                      Hd, m->gb2.p);
   gemm(st, m->dZ2b.p, Hd, 1, m->W2b.p, 1, Hd, m->dZ1.p, Hd, n, Hd, Hd, nullptr, 0, nullptr, 0.f);
   hipLaunchKernelGGL(relu_mask_kernel, dim3(grid1((int64_t)n * Hd)), dim3(256), 0, st, m->dZ1.p, m->Z1.p,
                      (int64_t)n * Hd, m->dZ1b.p);
   // layer 1: dW1 = X^T dZ1 (D x H); dX = dZ1 W1^T (n x D)
-  gemm(st, m->Xb.p, 1, D, m->dZ1b.p, Hd, 1, m->gW1.p, Hd, D, Hd, n, nullptr, 0, nullptr, 0.f);
+  gemm(st, m->Xb.p, 1, D, m->dZ1b.p, Hd, 1, m->gW1.p, Hd, D, Hd, n, nullptr, 0, nullptr, 0.f, &m->ws);
   hipLaunchKernelGGL(colsum_kernel, dim3(colsum_grid(Hd)), dim3(256), 0, st, m->dZ1.p, (const float *)nullptr, n,
                      Hd, m->gb1.p);
   gemm(st, m->dZ1b.p, Hd, 1, m->W1b.p, 1, Hd, m->dX.p, D, n, D, Hd, nullptr, 0, nullptr, 0.f);
