@@ -75,6 +75,7 @@ def main():
                     help="determinants selected in one step before the S^2 completion (about 2^21 after it)")
     ap.add_argument("--sel-no-davidson", action="store_true",
                     help="selected-psi leg: sigma timings only (for counter runs)")
+    ap.add_argument("--no-mlp", action="store_true")
     ap.add_argument("--no-cipsi", action="store_true")
     ap.add_argument("--cipsi-ndet", type=int, default=32768)
     ap.add_argument("--calib-copy", action="store_true",
@@ -224,6 +225,8 @@ def main():
         out["selected_davidson"] = _guarded(selected_davidson_leg, args, log)
     if not args.no_pt2:
         out["pt2"] = _guarded(pt2_leg, args, log)
+    if not args.no_mlp:
+        out["mlp"] = _guarded(mlp_leg, args, log)
     if not args.no_cipsi:
         out["cipsi_step"] = _guarded(cipsi_step_leg, args, log)
     print(json.dumps(out), flush=True)
@@ -279,6 +282,50 @@ def davidson_leg(ctx, H, dets, log):
     res["workload"] = ("davidson_diag_hs2, lowest 3 roots of H (no S^2 filter: seeded random guess vectors are "
                        "not spin-adapted) + 1 extra vector, threshold 1e-10 on the squared residual, HF + seeded "
                        "random guess, same 9,018,009-determinant space")
+    return res
+
+
+MFMA_BF16_PEAK = 2.5e15   # MI355X dense bf16 FLOP/s
+
+
+def mlp_leg(args, log):
+    """Block-proposal network throughput: Adam training epochs (batch 2048)
+    and prediction on 262,144 synthetic block descriptors of the 96-orbital
+    system, against the dense bf16 MFMA peak."""
+    import numpy as np
+    from quantum_package_amd.api import BlockMLP
+    rng = np.random.default_rng(3)
+    n, M = 262144, 96
+    desc = np.stack([rng.integers(0, 10, n), rng.integers(0, 10, n), rng.integers(0, 3, n),
+                     rng.integers(0, 5, n)], axis=1).astype(np.int32)
+    fdesc = rng.standard_normal((n, 3)).astype(np.float32)
+    y = rng.standard_normal(n).astype(np.float32)
+    m = BlockMLP(n_orb=M, seed=7)
+    m.train_epochs(desc[:16384], fdesc[:16384], y[:16384], 1e-3, 1, 2048, 1)   # warm-up
+    macs = 80 * 256 + 256 * 256 * 2 + 256            # per sample, forward
+    fwd = 2.0 * macs
+    res = {}
+    for batch in (2048, 16384):
+        t = time.time()
+        m.train_epochs(desc, fdesc, y, 1e-3, 2, batch, 2)
+        dt = time.time() - t
+        steps = 2 * (n // batch)
+        fl = 3.0 * fwd * batch * steps
+        res[f"train_batch{batch}"] = {"seconds": dt, "ms_per_step": 1e3 * dt / steps,
+                                      "samples_per_s": batch * steps / dt, "tflops": fl / dt / 1e12}
+        log(f"bench: mlp training, batch {batch}: {1e3 * dt / steps:.3f} ms/step, {fl / dt / 1e12:.2f} TFLOP/s")
+    t = time.time()
+    m.predict(desc, fdesc)
+    dt = time.time() - t
+    res["predict"] = {"seconds": dt, "samples_per_s": n / dt, "tflops": fwd * n / dt / 1e12,
+                      "note": "host descriptors in, predictions out (chunks of 65536), transfers included"}
+    log(f"bench: mlp prediction of {n} blocks: {1e3 * dt:.1f} ms ({fwd * n / dt / 1e12:.2f} TFLOP/s)")
+    best = max(res["train_batch2048"]["tflops"], res["train_batch16384"]["tflops"])
+    res["roofline"] = {"bound": "mfma", "achieved": best, "peak": MFMA_BF16_PEAK / 1e12, "unit": "TFLOP/s",
+                       "frac": best * 1e12 / MFMA_BF16_PEAK}
+    res["workload"] = ("x(80) -> 256 -> 256 -> 256 -> 1, BF16 weights, FP32 accumulation; 3.0e5 FLOP per block "
+                       "forward, x3 for a training step; 262,144 synthetic descriptors (96 orbitals)")
+    m.close()
     return res
 
 
