@@ -310,6 +310,18 @@ class BlockMLP:
         a = np.ascontiguousarray(value, dtype=np.float32).ravel()
         self._check(self.L.qpx_mlp_param(self.ptr, PARAM_NAMES.index(name), 0, 1, a.ctypes.data_as(_c_fp), a.size))
 
+    def _gemm_layout(self, A, B, ta, tb, split):
+        """C = A B through the network's GEMM with A passed transposed in
+        memory (ta) and/or B (tb), and the K split enabled (split)."""
+        M, K = A.shape
+        N = B.shape[1]
+        Ad = np.ascontiguousarray(A.T if ta else A, dtype=np.float32)
+        Bd = np.ascontiguousarray(B.T if tb else B, dtype=np.float32)
+        Cm = np.zeros((M, N), dtype=np.float32)
+        self._check(self.L.qpx_debug_mlp_gemm(self.ptr, M, N, K, Ad.ctypes.data_as(_c_fp), Bd.ctypes.data_as(_c_fp),
+                                              int(ta), int(tb), int(split), Cm.ctypes.data_as(_c_fp)))
+        return Cm
+
     def gemm_check(self, A, B):
         A = np.ascontiguousarray(A, dtype=np.float32)
         B = np.ascontiguousarray(B, dtype=np.float32)

@@ -16,7 +16,10 @@
 // strides (so A^T, B^T need no copies), C fp32; workgroup tile 64x64, four
 // wavefronts each owning 32x32 = 2x2 tiles of v_mfma_f32_16x16x16_bf16.
 // Lane layout of the 16x16x16 MFMA: lane l holds A[l%16][4(l/16)+0..3],
-// B[4(l/16)+0..3][l%16] and D[4(l/16)+0..3][l%16].
+// B[4(l/16)+0..3][l%16] and D[4(l/16)+0..3][l%16].  Both tiles sit in LDS
+// with k contiguous (A as [i][k], B as [j][k]), so each lane's MFMA operand
+// is one 8-byte read; they are staged in runs of 8 elements (16-byte loads)
+// along whichever index of the operand is contiguous in memory.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -30,6 +33,7 @@ namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline uint16_t f2bf(float x) {  // round to nearest even
   uint32_t u = __float_as_uint(x);
@@ -53,13 +57,55 @@ struct GemmArgs {
   float beta;         // C = acc + beta * C_old (gradient accumulation)
   float *part;        // split-K partial products (ksplit x M x N), or null
   int ksplit;         // K is cut into ksplit chunks (blockIdx.z); 1 = no split
+  int a_run, b_run;   // staging: 1 = runs along k, 2 = along i (A) / j (B), 0 = element by element
 };
 
-constexpr int TM = 64, TN = 64, TK = 16;
+// LDK pads a tile row to 80 bytes: rows stay 16-byte aligned for the staging
+// stores and 8-byte aligned for the operand reads
+constexpr int TM = 64, TN = 64, TK = 32, LDK = TK + 8;
+
+// Stages the rows x TK tile X(r, k) = P[r*rs + k*cs], r in [r0, r0+64),
+// k in [k0, k0+TK), into T[r - r0][k - k0]; entries outside r < rmax,
+// k < kmax are zero.  Each of the 256 threads moves one run of 8 elements:
+// run = 1 reads it along k (cs == 1), run = 2 along r (rs == 1), run = 0
+// element by element (any strides).  A run entirely in bounds is one
+// 16-byte load; the host picks run 1 or 2 only for 16-byte-aligned runs.
+__device__ inline void stage_tile(uint16_t (*T)[LDK], const uint16_t *__restrict__ P, int64_t rs, int64_t cs,
+                                  int run, int r0, int k0, int rmax, int kmax, int tid) {
+  u32x4 v = {0u, 0u, 0u, 0u};   // the run, two elements per word, element e in bits 16(e&1) of word e>>1
+  if (run == 2) {
+    const int kk = tid >> 3, rr = (tid & 7) * 8;
+    const int gk = k0 + kk, gr = r0 + rr;
+    if (gk < kmax) {
+      if (gr + 8 <= rmax) {
+        v = *(const u32x4 *)(P + (int64_t)gk * cs + gr);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          if (gr + e < rmax) v[e >> 1] |= (uint32_t)P[(int64_t)gk * cs + gr + e] << (16 * (e & 1));
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) T[rr + e][kk] = (uint16_t)(v[e >> 1] >> (16 * (e & 1)));
+  } else {
+    const int rr = tid >> 2, kk = (tid & 3) * 8;
+    const int gr = r0 + rr, gk = k0 + kk;
+    if (gr < rmax) {
+      if (run == 1 && gk + 8 <= kmax) {
+        v = *(const u32x4 *)(P + (int64_t)gr * rs + gk);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          if (gk + e < kmax) v[e >> 1] |= (uint32_t)P[(int64_t)gr * rs + (int64_t)(gk + e) * cs] << (16 * (e & 1));
+      }
+    }
+    *(u32x4 *)&T[rr][kk] = v;
+  }
+}
 
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
-  __shared__ uint16_t As[TK][TM + 4];  // k-major
-  __shared__ uint16_t Bs[TK][TN + 4];
+  __shared__ __align__(16) uint16_t As[TM][LDK];   // As[i][k]
+  __shared__ __align__(16) uint16_t Bs[TN][LDK];   // Bs[j][k]
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int i0 = blockIdx.y * TM, j0 = blockIdx.x * TN;
   const int wi = (wv >> 1) * 32, wj = (wv & 1) * 32;
@@ -72,41 +118,28 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
   // whole multiples of TK
   const int kc = ((g.K + g.ksplit - 1) / g.ksplit + TK - 1) / TK * TK;
   const int k_lo = (int)blockIdx.z * kc, k_hi = min(g.K, k_lo + kc);
+  const int r = lane & 15, q = (lane >> 4) * 4;
   for (int k0 = k_lo; k0 < k_hi; k0 += TK) {
     __syncthreads();
-    // stage A tile (TM x TK) and B tile (TK x TN): 1024 elements each
-    for (int x = tid; x < TM * TK; x += 256) {
-      int ii = x / TK, kk = x - ii * TK;
-      int gi = i0 + ii, gk = k0 + kk;
-      As[kk][ii] = (gi < g.M && gk < k_hi) ? g.A[gi * g.a_rs + gk * g.a_cs] : (uint16_t)0;
-    }
-    for (int x = tid; x < TK * TN; x += 256) {
-      int kk = x / TN, jj = x - kk * TN;
-      int gk = k0 + kk, gj = j0 + jj;
-      Bs[kk][jj] = (gk < k_hi && gj < g.N) ? g.B[gk * g.b_rs + gj * g.b_cs] : (uint16_t)0;
-    }
+    // B(k, j) = B[k*b_rs + j*b_cs] is staged as the tile X(j, k)
+    stage_tile(As, g.A, g.a_rs, g.a_cs, g.a_run, i0, k0, g.M, k_hi, tid);
+    stage_tile(Bs, g.B, g.b_cs, g.b_rs, g.b_run, j0, k0, g.N, k_hi, tid);
     __syncthreads();
-    const int r = lane & 15, q = (lane >> 4) * 4;
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      bf16x4 av;
-      av[0] = (short)As[q + 0][wi + 16 * a + r];
-      av[1] = (short)As[q + 1][wi + 16 * a + r];
-      av[2] = (short)As[q + 2][wi + 16 * a + r];
-      av[3] = (short)As[q + 3][wi + 16 * a + r];
+    for (int ks = 0; ks < TK; ks += 16) {
+      bf16x4 av[2], bv[2];
 #pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        bf16x4 bv;
-        bv[0] = (short)Bs[q + 0][wj + 16 * b + r];
-        bv[1] = (short)Bs[q + 1][wj + 16 * b + r];
-        bv[2] = (short)Bs[q + 2][wj + 16 * b + r];
-        bv[3] = (short)Bs[q + 3][wj + 16 * b + r];
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[a][b], 0, 0, 0);
-      }
+      for (int a = 0; a < 2; ++a) av[a] = *(const bf16x4 *)&As[wi + 16 * a + r][ks + q];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) bv[b] = *(const bf16x4 *)&Bs[wj + 16 * b + r][ks + q];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[a], bv[b], acc[a][b], 0, 0, 0);
     }
   }
   // epilogue: lane holds D[q+0..3][r] of each 16x16 tile
-  const int r = lane & 15, q = (lane >> 4) * 4;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -300,11 +333,21 @@ namespace {
 void gemm(hipStream_t st, const uint16_t *A, int64_t ars, int64_t acs, const uint16_t *B, int64_t brs, int64_t bcs,
           float *C, int64_t crs, int M, int N, int K, const float *bias, int relu, uint16_t *Cb, float beta,
           DBuf<float> *ws = nullptr) {
-  GemmArgs g{A, ars, acs, B, brs, bcs, C, crs, M, N, K, bias, relu, Cb, beta, nullptr, 1};
+  // 16-byte runs need a 16-byte-aligned base and a multiple-of-8 stride
+  // across runs
+  const auto run_of = [](const uint16_t *P, int64_t rs, int64_t cs) {
+    if (((uintptr_t)P & 15) != 0) return 0;
+    if (cs == 1 && rs % 8 == 0) return 1;
+    if (rs == 1 && cs % 8 == 0) return 2;
+    return 0;
+  };
+  GemmArgs g{A, ars, acs, B, brs, bcs, C, crs, M, N, K, bias, relu, Cb, beta, nullptr, 1,
+             run_of(A, ars, acs), run_of(B, bcs, brs)};
   dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
   const int tiles = (int)(grid.x * grid.y);
-  if (ws && tiles < 128 && K >= 8 * TK) {
-    g.ksplit = std::max(1, std::min({256 / tiles, K / (4 * TK), 32}));
+  constexpr int kGrain = 16;   // split-K sizing unit (slices are then rounded up to TK)
+  if (ws && tiles < 128 && K >= 8 * kGrain) {
+    g.ksplit = std::max(1, std::min({256 / tiles, K / (4 * kGrain), 32}));
     if (g.ksplit > 1 && ws->n >= (size_t)g.ksplit * M * N) {
       g.part = ws->p;
       grid.z = g.ksplit;
@@ -642,6 +685,34 @@ extern "C" int qpx_mlp_param(qpx_mlp *m, int which, int grad, int set, float *da
 
 // Plain bf16 GEMM C = A B (row-major A n x k, B k x m, fp32 inputs rounded
 // to bf16), for the lane-layout and accuracy checks of the MFMA tiles.
+// Test entry: C = A B through the network's GEMM in every operand layout it
+// is called with.  ta = 0: A given (M x K) row-major; ta = 1: A given as its
+// transpose, (K x M) row-major.  tb = 0: B (K x N) row-major; tb = 1: (N x K).
+// split = 1 passes a workspace, so the K split runs where it would for a
+// weight gradient.
+extern "C" int qpx_debug_mlp_gemm(qpx_mlp *m, int M, int N, int K, const float *A, const float *B, int ta, int tb,
+                                  int split, float *C) {
+  if (M < 1 || N < 1 || K < 1 || !A || !B || !C) return 1;
+  return mguard(m, [&] {
+    QPX_HIP(hipSetDevice(m->device));
+    DBuf<float> a, b, c, ws;
+    DBuf<uint16_t> ab, bb;
+    a.upload(A, (size_t)M * K, m->st);
+    b.upload(B, (size_t)K * N, m->st);
+    ab.alloc((size_t)M * K);
+    bb.alloc((size_t)K * N);
+    c.alloc((size_t)M * N);
+    if (split) ws.alloc((size_t)32 * M * N);
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(64), dim3(256), 0, m->st, a.p, ab.p, (int64_t)M * K);
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(64), dim3(256), 0, m->st, b.p, bb.p, (int64_t)K * N);
+    const int64_t ars = ta ? 1 : K, acs = ta ? M : 1;
+    const int64_t brs = tb ? 1 : N, bcs = tb ? K : 1;
+    gemm(m->st, ab.p, ars, acs, bb.p, brs, bcs, c.p, N, M, N, K, nullptr, 0, nullptr, 0.f, split ? &ws : nullptr);
+    QPX_HIP(hipMemcpyAsync(C, c.p, sizeof(float) * M * N, hipMemcpyDeviceToHost, m->st));
+    QPX_HIP(hipStreamSynchronize(m->st));
+  });
+}
+
 extern "C" int qpx_mlp_gemm_check(qpx_mlp *m, int M, int N, int K, const float *A, const float *B, float *C) {
   return mguard(m, [&] {
     QPX_HIP(hipSetDevice(m->device));

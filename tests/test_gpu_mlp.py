@@ -37,6 +37,23 @@ def test_mfma_gemm_layout(mlp, M, N, K):
     assert np.max(np.abs(C - want)) <= 1e-5 * np.max(np.abs(want)) + 1e-6
 
 
+@pytest.mark.parametrize("M,N,K,split", [(70, 90, 50, 0), (64, 64, 32, 0), (256, 256, 256, 0), (1000, 256, 80, 0),
+                                         (256, 256, 2048, 1), (100, 130, 700, 1), (256, 1, 700, 1)])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_mfma_gemm_operand_layouts(mlp, M, N, K, split, ta, tb):
+    """Every operand layout the network uses (A^T for the weight gradients,
+    B^T for the input gradients), aligned and unaligned leading dimensions,
+    ragged tiles, and the K split with its ordered reduction."""
+    rng = np.random.default_rng(7 * M + 5 * N + K + 3 * ta + tb)
+    A = rng.standard_normal((M, K)).astype(np.float32)
+    B = rng.standard_normal((K, N)).astype(np.float32)
+    _say(f"[mlp gemm layout] {M}x{N}x{K} ta={ta} tb={tb} split={split}")
+    C = mlp._gemm_layout(A, B, ta, tb, split)
+    want = mlp_ref.bf16(A).astype(np.float64) @ mlp_ref.bf16(B).astype(np.float64)
+    # fp32 accumulation over K terms of bf16 products
+    assert np.max(np.abs(C - want)) <= 1e-6 * np.sqrt(K) * np.max(np.abs(want)) + 1e-5
+
+
 def _random_batch(n, n_orb, rng):
     desc = np.zeros((n, 4), dtype=np.int32)
     desc[:, 0] = rng.integers(0, n_orb, n)
