@@ -275,6 +275,34 @@ __global__ __launch_bounds__(256) void colsum_kernel(const float *__restrict__ A
 
 static inline int colsum_grid(int m) { return m < 32 ? 1 : (m + 31) / 32; }
 
+// Column sums of a wide matrix in two passes, each in a fixed order:
+// workgroup (column block b, row chunk p) sums 64 columns over the rows of
+// its chunk (4 row phases per column, combined in phase order) into
+// part[p][k]; the second pass adds the chunks in chunk order.  Spreading the
+// rows over many workgroups is what fills the GPU: one workgroup per 32
+// columns left all but 8 CUs idle for m = 256.
+__global__ __launch_bounds__(256) void colsum_part_kernel(const float *__restrict__ A, const float *__restrict__ w,
+                                                          int n, int m, int chunk, float *__restrict__ part) {
+  __shared__ float red[4][64];
+  const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int k = blockIdx.x * 64 + c, p = blockIdx.y;
+  const int r0 = p * chunk, r1 = min(n, r0 + chunk);
+  float s = 0.f;
+  if (k < m)
+    for (int i = r0 + g; i < r1; i += 4) s += A[(int64_t)i * m + k] * (w ? w[i] : 1.f);
+  red[g][c] = s;
+  __syncthreads();
+  if (g == 0 && k < m) part[(int64_t)p * m + k] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
+}
+
+__global__ void colsum_final_kernel(const float *__restrict__ part, int np, int m, float *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  float s = 0.f;
+  for (int p = 0; p < np; ++p) s += part[(int64_t)p * m + k];
+  out[k] = s;
+}
+
 __global__ void emb_grad_kernel(const int *__restrict__ desc, const float *__restrict__ dX, int n, int din,
                                 float *__restrict__ demb) {
   for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < (int64_t)n * 64;
@@ -365,6 +393,25 @@ void gemm(hipStream_t st, const uint16_t *A, int64_t ars, int64_t acs, const uin
   }
 }
 
+// out[k] = sum_i A[i][k] w[i] (w null: 1) over the n x mc row-major A; the
+// wide case uses the workspace for its per-chunk partial sums
+void colsum(hipStream_t st, const float *A, const float *w, int n, int mc, float *out, DBuf<float> &ws) {
+  if (mc < 32) {
+    hipLaunchKernelGGL(colsum_kernel, dim3(colsum_grid(mc)), dim3(256), 0, st, A, w, n, mc, out);
+    QPX_HIP(hipGetLastError());
+    return;
+  }
+  const int nb = (mc + 63) / 64;
+  int np = std::max(1, std::min(256 / nb, (n + 63) / 64));
+  np = (int)std::max<int64_t>(1, std::min<int64_t>(np, (int64_t)(ws.n / (size_t)mc)));
+  const int chunk = std::max(1, (n + np - 1) / np);
+  np = std::max(1, (n + chunk - 1) / chunk);
+  hipLaunchKernelGGL(colsum_part_kernel, dim3(nb, np), dim3(256), 0, st, A, w, n, mc, chunk, ws.p);
+  hipLaunchKernelGGL(colsum_final_kernel, dim3((mc + 255) / 256), dim3(256), 0, st, (const float *)ws.p, np, mc,
+                     out);
+  QPX_HIP(hipGetLastError());
+}
+
 void ensure(qpx_mlp *m, int n) {
   if (n <= m->cap) return;
   int c = n;
@@ -400,29 +447,25 @@ float train_step(qpx_mlp *m, int n, float lr) {
   hipLaunchKernelGGL(head_kernel, dim3(nb), dim3(256), 0, st, m->Z3.p, m->w4.p, m->b4, n, Hd, m->tgt.p, m->y.p,
                      m->dy.p, m->lossp.p);
   // head grads
-  hipLaunchKernelGGL(colsum_kernel, dim3(colsum_grid(Hd)), dim3(256), 0, st, m->Z3.p, m->dy.p, n, Hd, m->gw4.p);
-  hipLaunchKernelGGL(colsum_kernel, dim3(colsum_grid(1)), dim3(256), 0, st, m->dy.p, (const float *)nullptr, n, 1,
-                     m->gb4.p);
+  colsum(st, m->Z3.p, m->dy.p, n, Hd, m->gw4.p, m->ws);
+  colsum(st, m->dy.p, nullptr, n, 1, m->gb4.p, m->ws);
   hipLaunchKernelGGL(head_back_kernel, dim3(grid1((int64_t)n * Hd)), dim3(256), 0, st, m->Z3.p, m->w4.p, m->dy.p,
                      n, Hd, m->dZ3.p, m->dZ3b.p);
   // layer 3: dW3 = H2^T dZ3 (H x H, K = n); db3 = colsum dZ3; dH2 = dZ3 W3^T
   gemm(st, m->H2b.p, 1, Hd, m->dZ3b.p, Hd, 1, m->gW3.p, Hd, Hd, Hd, n, nullptr, 0, nullptr, 0.f, &m->ws);
-  hipLaunchKernelGGL(colsum_kernel, dim3(colsum_grid(Hd)), dim3(256), 0, st, m->dZ3.p, (const float *)nullptr, n,
-                     Hd, m->gb3.p);
+  colsum(st, m->dZ3.p, nullptr, n, Hd, m->gb3.p, m->ws);
   gemm(st, m->dZ3b.p, Hd, 1, m->W3b.p, 1, Hd, m->dZ2.p, Hd, n, Hd, Hd, nullptr, 0, nullptr, 0.f);
   hipLaunchKernelGGL(relu_mask_kernel, dim3(grid1((int64_t)n * Hd)), dim3(256), 0, st, m->dZ2.p, m->Z2.p,
                      (int64_t)n * Hd, m->dZ2b.p);
   // layer 2
   gemm(st, m->H1b.p, 1, Hd, m->dZ2b.p, Hd, 1, m->gW2.p, Hd, Hd, Hd, n, nullptr, 0, nullptr, 0.f, &m->ws);
-  hipLaunchKernelGGL(colsum_kernel, dim3(colsum_grid(Hd)), dim3(256), 0, st, m->dZ2.p, (const float *)nullptr, n,
-                     Hd, m->gb2.p);
+  colsum(st, m->dZ2.p, nullptr, n, Hd, m->gb2.p, m->ws);
   gemm(st, m->dZ2b.p, Hd, 1, m->W2b.p, 1, Hd, m->dZ1.p, Hd, n, Hd, Hd, nullptr, 0, nullptr, 0.f);
   hipLaunchKernelGGL(relu_mask_kernel, dim3(grid1((int64_t)n * Hd)), dim3(256), 0, st, m->dZ1.p, m->Z1.p,
                      (int64_t)n * Hd, m->dZ1b.p);
   // layer 1: dW1 = X^T dZ1 (D x H); dX = dZ1 W1^T (n x D)
   gemm(st, m->Xb.p, 1, D, m->dZ1b.p, Hd, 1, m->gW1.p, Hd, D, Hd, n, nullptr, 0, nullptr, 0.f, &m->ws);
-  hipLaunchKernelGGL(colsum_kernel, dim3(colsum_grid(Hd)), dim3(256), 0, st, m->dZ1.p, (const float *)nullptr, n,
-                     Hd, m->gb1.p);
+  colsum(st, m->dZ1.p, nullptr, n, Hd, m->gb1.p, m->ws);
   gemm(st, m->dZ1b.p, Hd, 1, m->W1b.p, 1, Hd, m->dX.p, D, n, D, Hd, nullptr, 0, nullptr, 0.f);
   QPX_HIP(hipMemsetAsync(m->gemb.p, 0, sizeof(float) * m->emb.n, st));
   hipLaunchKernelGGL(emb_grad_kernel, dim3(grid1((int64_t)n * 64)), dim3(256), 0, st, m->desc.p, m->dX.p, n, D,
