@@ -203,29 +203,37 @@ __global__ void features_kernel(const int *__restrict__ desc, const float *__res
 }
 
 // y = H3 w4 + b4 (one output), loss partials and dy = 2 (y - t) / n
-__global__ void head_kernel(const float *__restrict__ H, const float *__restrict__ w4, float b4, int n, int hdim,
-                            const float *__restrict__ t, float *__restrict__ y, float *__restrict__ dy,
-                            float *__restrict__ loss_part) {
-  __shared__ float red[256];
+// y = H w4 + b4 per row, one wavefront per row (the row read coalesced,
+// reduced by shuffles in a fixed order); with targets, dy = 2 (y - t) / n and
+// one partial sum of (y - t)^2 per workgroup
+__global__ __launch_bounds__(256) void head_kernel(const float *__restrict__ H, const float *__restrict__ w4, float b4,
+                                                   int n, int hdim, const float *__restrict__ t,
+                                                   float *__restrict__ y, float *__restrict__ dy,
+                                                   float *__restrict__ loss_part) {
+  __shared__ float red[4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   float l = 0.f;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    float s = b4;
-    for (int k = 0; k < hdim; ++k) s += H[(int64_t)i * hdim + k] * w4[k];
-    y[i] = s;
-    if (t) {
-      float e = s - t[i];
-      l += e * e;
-      if (dy) dy[i] = 2.f * e / n;
+  for (int i = blockIdx.x * 4 + wv; i < n; i += gridDim.x * 4) {
+    float s = 0.f;
+    for (int k = lane; k < hdim; k += 64) s += H[(int64_t)i * hdim + k] * w4[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    s += b4;
+    if (lane == 0) {
+      y[i] = s;
+      if (t) {
+        const float e = s - t[i];
+        l += e * e;
+        if (dy) dy[i] = 2.f * e / n;
+      }
     }
   }
-  red[threadIdx.x] = l;
+  if (lane == 0) red[wv] = l;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0 && loss_part) loss_part[blockIdx.x] = red[0];
+  if (threadIdx.x == 0 && loss_part) loss_part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
+
+static inline int head_blocks(int n) { return std::max(1, std::min(1024, (n + 3) / 4)); }
 
 // dZ = (dy w4^T) masked by H > 0; bf16 copy; dw4 = H^T dy partial sums
 __global__ void head_back_kernel(const float *__restrict__ H, const float *__restrict__ w4,
@@ -443,7 +451,7 @@ float train_step(qpx_mlp *m, int n, float lr) {
   hipStream_t st = m->st;
   const int D = m->din, Hd = m->hid;
   forward(m, n);
-  int nb = std::min(1024, (n + 255) / 256);
+  const int nb = head_blocks(n);
   hipLaunchKernelGGL(head_kernel, dim3(nb), dim3(256), 0, st, m->Z3.p, m->w4.p, m->b4, n, Hd, m->tgt.p, m->y.p,
                      m->dy.p, m->lossp.p);
   // head grads
@@ -681,7 +689,7 @@ extern "C" int qpx_mlp_predict(qpx_mlp *m, int n, const int *desc, const float *
       QPX_HIP(hipMemcpyAsync(m->desc.p, desc + 4 * (size_t)i0, sizeof(int) * 4 * c, hipMemcpyHostToDevice, m->st));
       QPX_HIP(hipMemcpyAsync(m->fdesc.p, fdesc + 3 * (size_t)i0, sizeof(float) * 3 * c, hipMemcpyHostToDevice, m->st));
       forward(m, c);
-      int nb = std::min(1024, (c + 255) / 256);
+      const int nb = head_blocks(c);
       hipLaunchKernelGGL(head_kernel, dim3(nb), dim3(256), 0, m->st, m->Z3.p, m->w4.p, m->b4, c, m->hid,
                          (const float *)nullptr, m->y.p, (float *)nullptr, (float *)nullptr);
       QPX_HIP(hipGetLastError());
