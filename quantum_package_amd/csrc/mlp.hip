@@ -206,7 +206,8 @@ __global__ void features_kernel(const int *__restrict__ desc, const float *__res
 // y = H w4 + b4 per row, one wavefront per row (the row read coalesced,
 // reduced by shuffles in a fixed order); with targets, dy = 2 (y - t) / n and
 // one partial sum of (y - t)^2 per workgroup
-__global__ __launch_bounds__(256) void head_kernel(const float *__restrict__ H, const float *__restrict__ w4, float b4,
+__global__ __launch_bounds__(256) void head_kernel(const float *__restrict__ H, const float *__restrict__ w4,
+                                                   const float *__restrict__ b4p,
                                                    int n, int hdim, const float *__restrict__ t,
                                                    float *__restrict__ y, float *__restrict__ dy,
                                                    float *__restrict__ loss_part) {
@@ -218,7 +219,7 @@ __global__ __launch_bounds__(256) void head_kernel(const float *__restrict__ H, 
     for (int k = lane; k < hdim; k += 64) s += H[(int64_t)i * hdim + k] * w4[k];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    s += b4;
+    s += *b4p;
     if (lane == 0) {
       y[i] = s;
       if (t) {
@@ -234,6 +235,21 @@ __global__ __launch_bounds__(256) void head_kernel(const float *__restrict__ H, 
 }
 
 static inline int head_blocks(int n) { return std::max(1, std::min(1024, (n + 3) / 4)); }
+
+// out[0] = (sum of the nb loss partials, in a fixed order) / n
+__global__ __launch_bounds__(256) void loss_reduce_kernel(const float *__restrict__ part, int nb, int n,
+                                                          float *__restrict__ out) {
+  __shared__ float red[256];
+  float s = 0.f;
+  for (int i = threadIdx.x; i < nb; i += 256) s += part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0] / (float)n;
+}
 
 // dZ = (dy w4^T) masked by H > 0; bf16 copy; dw4 = H^T dy partial sums
 __global__ void head_back_kernel(const float *__restrict__ H, const float *__restrict__ w4,
@@ -350,7 +366,7 @@ struct qpx_mlp {
   DBuf<float> gW1, gW2, gW3, gb1, gb2, gb3, gw4, gemb, gb4;
   DBuf<float> ws;   // split-K partial products of the weight gradients (sized once)
   DBuf<float> mW1, mW2, mW3, mb1, mb2, mb3, mw4, memb, vW1, vW2, vW3, vb1, vb2, vb3, vw4, vemb;
-  float b4 = 0.f, mb4 = 0.f, vb4 = 0.f;
+  DBuf<float> b4, mb4, vb4;   // the output bias and its Adam moments (one element each, on the device)
   int step = 0;
   // activations for a batch
   int cap = 0;
@@ -428,7 +444,7 @@ void ensure(qpx_mlp *m, int n) {
   m->H1b.alloc((size_t)c * m->hid); m->H2b.alloc((size_t)c * m->hid); m->H3b.alloc((size_t)c * m->hid);
   m->dZ1.alloc((size_t)c * m->hid); m->dZ2.alloc((size_t)c * m->hid); m->dZ3.alloc((size_t)c * m->hid);
   m->dZ1b.alloc((size_t)c * m->hid); m->dZ2b.alloc((size_t)c * m->hid); m->dZ3b.alloc((size_t)c * m->hid);
-  m->y.alloc(c); m->dy.alloc(c); m->lossp.alloc(1024);
+  m->y.alloc(c); m->dy.alloc(c); m->lossp.alloc(1025);   // head partials + one loss slot
   m->desc.alloc((size_t)4 * c); m->fdesc.alloc((size_t)3 * c); m->tgt.alloc(c);
   if (!m->ws.p) m->ws.alloc((size_t)32 * m->hid * m->hid);   // the largest split: 32 x hid x hid
   m->cap = c;
@@ -447,12 +463,15 @@ void forward(qpx_mlp *m, int n) {
   gemm(st, m->H2b.p, Hd, 1, m->W3b.p, Hd, 1, m->Z3.p, Hd, n, Hd, Hd, m->b3.p, 1, m->H3b.p, 0.f);
 }
 
-float train_step(qpx_mlp *m, int n, float lr) {
+// One Adam step on the batch in place, enqueued on the network's stream
+// without waiting for it; the batch's mean squared error goes to *loss_out
+// (device memory).
+void train_step_async(qpx_mlp *m, int n, float lr, float *loss_out) {
   hipStream_t st = m->st;
   const int D = m->din, Hd = m->hid;
   forward(m, n);
   const int nb = head_blocks(n);
-  hipLaunchKernelGGL(head_kernel, dim3(nb), dim3(256), 0, st, m->Z3.p, m->w4.p, m->b4, n, Hd, m->tgt.p, m->y.p,
+  hipLaunchKernelGGL(head_kernel, dim3(nb), dim3(256), 0, st, m->Z3.p, m->w4.p, m->b4.p, n, Hd, m->tgt.p, m->y.p,
                      m->dy.p, m->lossp.p);
   // head grads
   colsum(st, m->Z3.p, m->dy.p, n, Hd, m->gw4.p, m->ws);
@@ -495,20 +514,21 @@ float train_step(qpx_mlp *m, int n, float lr) {
     ad(m->b3, m->gb3, m->mb3, m->vb3, nullptr);
     ad(m->w4, m->gw4, m->mw4, m->vw4, nullptr);
     ad(m->emb, m->gemb, m->memb, m->vemb, nullptr);
+    ad(m->b4, m->gb4, m->mb4, m->vb4, nullptr);
     QPX_HIP(hipGetLastError());
-    float gb4 = 0.f;
-    QPX_HIP(hipMemcpyAsync(&gb4, m->gb4.p, sizeof(float), hipMemcpyDeviceToHost, st));
-    QPX_HIP(hipStreamSynchronize(st));
-    m->mb4 = b1 * m->mb4 + (1 - b1) * gb4;
-    m->vb4 = b2 * m->vb4 + (1 - b2) * gb4 * gb4;
-    m->b4 -= lr * (m->mb4 / c1) / (std::sqrt(m->vb4 / c2) + eps);
   }
-  std::vector<float> lp(nb);
-  QPX_HIP(hipMemcpyAsync(lp.data(), m->lossp.p, sizeof(float) * nb, hipMemcpyDeviceToHost, st));
-  QPX_HIP(hipStreamSynchronize(st));
-  double l = 0.0;
-  for (float x : lp) l += x;
-  return (float)(l / n);
+  // the step's mean squared error, left on the device
+  hipLaunchKernelGGL(loss_reduce_kernel, dim3(1), dim3(256), 0, st, m->lossp.p, nb, n, loss_out);
+  QPX_HIP(hipGetLastError());
+}
+
+// one step with its loss returned (synchronises the stream)
+float train_step(qpx_mlp *m, int n, float lr) {
+  train_step_async(m, n, lr, m->lossp.p + 1024);
+  float l = 0.f;
+  QPX_HIP(hipMemcpyAsync(&l, m->lossp.p + 1024, sizeof(float), hipMemcpyDeviceToHost, m->st));
+  QPX_HIP(hipStreamSynchronize(m->st));
+  return l;
 }
 
 // batch rows of the device-resident training set, in permutation order
@@ -590,6 +610,7 @@ extern "C" int qpx_mlp_create(int device, int n_orb, uint64_t seed, qpx_mlp **ou
     };
     zero(m->gW1, (size_t)D * Hd); zero(m->gW2, (size_t)Hd * Hd); zero(m->gW3, (size_t)Hd * Hd);
     zero(m->gb1, Hd); zero(m->gb2, Hd); zero(m->gb3, Hd); zero(m->gw4, Hd); zero(m->gb4, 1);
+    zero(m->b4, 1); zero(m->mb4, 1); zero(m->vb4, 1);
     zero(m->gemb, (size_t)n_orb * 32);
     zero(m->mW1, (size_t)D * Hd); zero(m->mW2, (size_t)Hd * Hd); zero(m->mW3, (size_t)Hd * Hd);
     zero(m->vW1, (size_t)D * Hd); zero(m->vW2, (size_t)Hd * Hd); zero(m->vW3, (size_t)Hd * Hd);
@@ -655,6 +676,10 @@ extern "C" int qpx_mlp_train_epochs(qpx_mlp *m, int n, const int *desc, const fl
     fdesc_all.upload(fdesc, (size_t)3 * n, m->st);
     tgt_all.upload(target, (size_t)n, m->st);
     perm_d.alloc(n);
+    const int steps = n / batch;
+    DBuf<float> step_loss;   // one loss per step of an epoch, read back once per epoch
+    step_loss.alloc(steps);
+    std::vector<float> sl(steps);
     std::vector<int> perm(n);
     uint64_t s = seed;
     for (int ep = 0; ep < epochs; ++ep) {
@@ -664,16 +689,18 @@ extern "C" int qpx_mlp_train_epochs(qpx_mlp *m, int n, const int *desc, const fl
         std::swap(perm[i], perm[j]);
       }
       QPX_HIP(hipMemcpyAsync(perm_d.p, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, m->st));
-      double lsum = 0.0;
-      int steps = 0;
-      for (int b0 = 0; b0 + batch <= n; b0 += batch) {
-        hipLaunchKernelGGL(gather_batch_kernel, dim3((batch + 255) / 256), dim3(256), 0, m->st, perm_d.p + b0, batch,
-                           desc_all.p, fdesc_all.p, tgt_all.p, m->desc.p, m->fdesc.p, m->tgt.p);
+      for (int q = 0; q < steps; ++q) {
+        hipLaunchKernelGGL(gather_batch_kernel, dim3((batch + 255) / 256), dim3(256), 0, m->st,
+                           perm_d.p + (size_t)q * batch, batch, desc_all.p, fdesc_all.p, tgt_all.p, m->desc.p,
+                           m->fdesc.p, m->tgt.p);
         QPX_HIP(hipGetLastError());
-        lsum += train_step(m, batch, lr);
-        ++steps;
+        train_step_async(m, batch, lr, step_loss.p + q);
       }
-      epoch_loss[ep] = (float)(lsum / std::max(steps, 1));
+      QPX_HIP(hipMemcpyAsync(sl.data(), step_loss.p, sizeof(float) * steps, hipMemcpyDeviceToHost, m->st));
+      QPX_HIP(hipStreamSynchronize(m->st));
+      double lsum = 0.0;
+      for (float x : sl) lsum += x;
+      epoch_loss[ep] = (float)(lsum / steps);
     }
   });
 }
@@ -690,7 +717,7 @@ extern "C" int qpx_mlp_predict(qpx_mlp *m, int n, const int *desc, const float *
       QPX_HIP(hipMemcpyAsync(m->fdesc.p, fdesc + 3 * (size_t)i0, sizeof(float) * 3 * c, hipMemcpyHostToDevice, m->st));
       forward(m, c);
       const int nb = head_blocks(c);
-      hipLaunchKernelGGL(head_kernel, dim3(nb), dim3(256), 0, m->st, m->Z3.p, m->w4.p, m->b4, c, m->hid,
+      hipLaunchKernelGGL(head_kernel, dim3(nb), dim3(256), 0, m->st, m->Z3.p, m->w4.p, m->b4.p, c, m->hid,
                          (const float *)nullptr, m->y.p, (float *)nullptr, (float *)nullptr);
       QPX_HIP(hipGetLastError());
       QPX_HIP(hipMemcpyAsync(out + i0, m->y.p, sizeof(float) * c, hipMemcpyDeviceToHost, m->st));
@@ -713,9 +740,11 @@ extern "C" int qpx_mlp_param(qpx_mlp *m, int which, int grad, int set, float *da
         QPX_HIP(hipMemcpyAsync(data, m->gb4.p, sizeof(float), hipMemcpyDeviceToHost, m->st));
         QPX_HIP(hipStreamSynchronize(m->st));
       } else if (set) {
-        m->b4 = data[0];
+        QPX_HIP(hipMemcpyAsync(m->b4.p, data, sizeof(float), hipMemcpyHostToDevice, m->st));
+        QPX_HIP(hipStreamSynchronize(m->st));
       } else {
-        data[0] = m->b4;
+        QPX_HIP(hipMemcpyAsync(data, m->b4.p, sizeof(float), hipMemcpyDeviceToHost, m->st));
+        QPX_HIP(hipStreamSynchronize(m->st));
       }
       return;
     }

@@ -87,6 +87,30 @@ def test_gradients_vs_reference(mlp):
         assert rel < 3e-2, k
 
 
+def test_train_epochs_matches_single_steps():
+    """The device-resident epoch loop (batches gathered on the device, the
+    output bias updated on the device, losses read once per epoch) against
+    the same full-batch Adam steps taken one call at a time.  Only the order
+    of the batch rows differs, so the sums round differently: 1e-4 on the
+    losses, 1e-3 on each parameter tensor."""
+    from quantum_package_amd.api import BlockMLP
+    rng = np.random.default_rng(21)
+    desc, fdesc, y = _random_batch(512, 30, rng)
+    a = BlockMLP(n_orb=30, seed=4)
+    b = BlockMLP(n_orb=30, seed=4)
+    _say("[mlp epochs] 3 full-batch epochs of 512 against 3 single steps")
+    la = a.train_epochs(desc, fdesc, y, 1e-3, 3, 512, seed=1)
+    lb = np.array([b.train_step(desc, fdesc, y, lr=1e-3) for _ in range(3)], dtype=np.float32)
+    np.testing.assert_allclose(la, lb, rtol=1e-4)
+    for k in ["W1", "W2", "W3", "b1", "b2", "b3", "w4", "emb", "b4"]:
+        pa, pb = a.get(k), b.get(k)
+        assert np.linalg.norm(pa - pb) <= 1e-3 * np.linalg.norm(pb) + 1e-6, k
+    # three Adam steps of lr = 1e-3 from zero move the bias, by at most ~3 lr
+    assert 0.0 < abs(float(a.get("b4").ravel()[0])) < 4e-3
+    a.close()
+    b.close()
+
+
 def test_training_reduces_loss():
     from quantum_package_amd.api import BlockMLP
     rng = np.random.default_rng(9)
