@@ -167,6 +167,8 @@ __global__ void gemm_splitk_reduce_kernel(GemmArgs g) {
   for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < mn; x += (int64_t)gridDim.x * blockDim.x) {
     const int gi = (int)(x / g.N), gj = (int)(x - (int64_t)gi * g.N);
     float v = 0.f;
+    // unrolled so that the slice loads are in flight together; the adds stay in slice order
+#pragma unroll 8
     for (int z = 0; z < g.ksplit; ++z) v += g.part[(int64_t)z * mn + x];
     float *c = g.C + gi * g.c_rs + gj;
     if (g.beta != 0.f) v += g.beta * (*c);
@@ -319,12 +321,18 @@ __global__ __launch_bounds__(256) void colsum_part_kernel(const float *__restric
   if (g == 0 && k < m) part[(int64_t)p * m + k] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
 }
 
-__global__ void colsum_final_kernel(const float *__restrict__ part, int np, int m, float *__restrict__ out) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m) return;
+// one wavefront per column: lane l adds chunks l, l + 64, ...; the lanes
+// are then combined by a fixed shuffle tree
+__global__ __launch_bounds__(256) void colsum_final_kernel(const float *__restrict__ part, int np, int m,
+                                                           float *__restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= m) return;   // whole wavefront
   float s = 0.f;
-  for (int p = 0; p < np; ++p) s += part[(int64_t)p * m + k];
-  out[k] = s;
+  for (int p = lane; p < np; p += 64) s += part[(int64_t)p * m + k];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) out[k] = s;
 }
 
 __global__ void emb_grad_kernel(const int *__restrict__ desc, const float *__restrict__ dX, int n, int din,
@@ -431,8 +439,7 @@ void colsum(hipStream_t st, const float *A, const float *w, int n, int mc, float
   const int chunk = std::max(1, (n + np - 1) / np);
   np = std::max(1, (n + chunk - 1) / chunk);
   hipLaunchKernelGGL(colsum_part_kernel, dim3(nb, np), dim3(256), 0, st, A, w, n, mc, chunk, ws.p);
-  hipLaunchKernelGGL(colsum_final_kernel, dim3((mc + 255) / 256), dim3(256), 0, st, (const float *)ws.p, np, mc,
-                     out);
+  hipLaunchKernelGGL(colsum_final_kernel, dim3((mc + 3) / 4), dim3(256), 0, st, (const float *)ws.p, np, mc, out);
   QPX_HIP(hipGetLastError());
 }
 
