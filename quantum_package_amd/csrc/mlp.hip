@@ -345,19 +345,37 @@ __global__ void emb_grad_kernel(const int *__restrict__ desc, const float *__res
   }
 }
 
-__global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
-                            float *__restrict__ v, int64_t n, float lr, float b1, float b2, float eps, float c1,
-                            float c2, uint16_t *__restrict__ pb) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    float gi = g[i];
-    float mi = b1 * m[i] + (1.f - b1) * gi;
-    float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-    m[i] = mi;
-    v[i] = vi;
-    float upd = lr * (mi / c1) / (sqrtf(vi / c2) + eps);
-    float pi = p[i] - upd;
-    p[i] = pi;
-    if (pb) pb[i] = f2bf(pi);
+// Adam over all parameter tensors in one launch: tensor q covers elements
+// [start[q], start[q+1]) of the concatenation.  Per element:
+// m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2,
+// p -= lr (m / c1) / (sqrt(v / c2) + eps), with c1, c2 the bias corrections
+// 1 - b1^t, 1 - b2^t; the bf16 copy of p is refreshed where there is one.
+constexpr int kAdamTensors = 9;
+struct AdamSet {
+  float *p[kAdamTensors];
+  const float *g[kAdamTensors];
+  float *m[kAdamTensors];
+  float *v[kAdamTensors];
+  uint16_t *pb[kAdamTensors];
+  int64_t start[kAdamTensors + 1];
+};
+
+__global__ void adam_multi_kernel(AdamSet S, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int64_t total = S.start[kAdamTensors];
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
+    int q = 0;
+#pragma unroll
+    for (int t = 1; t < kAdamTensors; ++t) q += (x >= S.start[t]);
+    const int64_t i = x - S.start[q];
+    const float gi = S.g[q][i];
+    const float mi = b1 * S.m[q][i] + (1.f - b1) * gi;
+    const float vi = b2 * S.v[q][i] + (1.f - b2) * gi * gi;
+    S.m[q][i] = mi;
+    S.v[q][i] = vi;
+    const float upd = lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    const float pi = S.p[q][i] - upd;
+    S.p[q][i] = pi;
+    if (S.pb[q]) S.pb[q][i] = f2bf(pi);
   }
 }
 
@@ -509,9 +527,16 @@ void train_step_async(qpx_mlp *m, int n, float lr, float *loss_out) {
     m->step++;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     const float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
+    AdamSet S{};
+    int nq = 0;
     auto ad = [&](DBuf<float> &p, DBuf<float> &g, DBuf<float> &mm, DBuf<float> &vv, uint16_t *pb) {
-      hipLaunchKernelGGL(adam_kernel, dim3(grid1((int64_t)p.n)), dim3(256), 0, st, p.p, g.p, mm.p, vv.p,
-                         (int64_t)p.n, lr, b1, b2, eps, c1, c2, pb);
+      S.p[nq] = p.p;
+      S.g[nq] = g.p;
+      S.m[nq] = mm.p;
+      S.v[nq] = vv.p;
+      S.pb[nq] = pb;
+      S.start[nq + 1] = S.start[nq] + (int64_t)p.n;
+      ++nq;
     };
     ad(m->W1, m->gW1, m->mW1, m->vW1, m->W1b.p);
     ad(m->W2, m->gW2, m->mW2, m->vW2, m->W2b.p);
@@ -522,6 +547,9 @@ void train_step_async(qpx_mlp *m, int n, float lr, float *loss_out) {
     ad(m->w4, m->gw4, m->mw4, m->vw4, nullptr);
     ad(m->emb, m->gemb, m->memb, m->vemb, nullptr);
     ad(m->b4, m->gb4, m->mb4, m->vb4, nullptr);
+    QPX_REQUIRE(nq == kAdamTensors, "Adam tensor count");
+    hipLaunchKernelGGL(adam_multi_kernel, dim3(grid1(S.start[kAdamTensors])), dim3(256), 0, st, S, lr, b1, b2, eps,
+                       c1, c2);
     QPX_HIP(hipGetLastError());
   }
   // the step's mean squared error, left on the device
