@@ -58,6 +58,7 @@ struct GemmArgs {
   float *part;        // split-K partial products (ksplit x M x N), or null
   int ksplit;         // K is cut into ksplit chunks (blockIdx.z); 1 = no split
   int a_run, b_run;   // staging: 1 = runs along k, 2 = along i (A) / j (B), 0 = element by element
+  const float *mask;  // optional: C(i,j) = 0 where mask(i,j) <= 0 (same layout as C; a ReLU's backward)
 };
 
 // LDK pads a tile row to 80 bytes: rows stay 16-byte aligned for the staging
@@ -155,6 +156,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
           if (g.beta != 0.f) v += g.beta * (*c);
           if (g.bias) v += g.bias[gj];
           if (g.relu) v = v > 0.f ? v : 0.f;
+          if (g.mask) v = g.mask[gi * g.c_rs + gj] > 0.f ? v : 0.f;
           *c = v;
           if (g.Cb) g.Cb[gi * g.c_rs + gj] = f2bf(v);
         }
@@ -174,6 +176,7 @@ __global__ void gemm_splitk_reduce_kernel(GemmArgs g) {
     if (g.beta != 0.f) v += g.beta * (*c);
     if (g.bias) v += g.bias[gj];
     if (g.relu) v = v > 0.f ? v : 0.f;
+    if (g.mask) v = g.mask[gi * g.c_rs + gj] > 0.f ? v : 0.f;
     *c = v;
     if (g.Cb) g.Cb[gi * g.c_rs + gj] = f2bf(v);
   }
@@ -263,15 +266,6 @@ __global__ void head_back_kernel(const float *__restrict__ H, const float *__res
     float v = H[x] > 0.f ? dy[i] * w4[k] : 0.f;
     dZ[x] = v;
     dZb[x] = f2bf(v);
-  }
-}
-
-__global__ void relu_mask_kernel(float *__restrict__ dH, const float *__restrict__ H, int64_t n,
-                                 uint16_t *__restrict__ dHb) {
-  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
-    float v = H[x] > 0.f ? dH[x] : 0.f;
-    dH[x] = v;
-    dHb[x] = f2bf(v);
   }
 }
 
@@ -410,7 +404,7 @@ namespace {
 // is cut along K across workgroups and reduced in slice order.
 void gemm(hipStream_t st, const uint16_t *A, int64_t ars, int64_t acs, const uint16_t *B, int64_t brs, int64_t bcs,
           float *C, int64_t crs, int M, int N, int K, const float *bias, int relu, uint16_t *Cb, float beta,
-          DBuf<float> *ws = nullptr) {
+          DBuf<float> *ws = nullptr, const float *mask = nullptr) {
   // 16-byte runs need a 16-byte-aligned base and a multiple-of-8 stride
   // across runs
   const auto run_of = [](const uint16_t *P, int64_t rs, int64_t cs) {
@@ -420,7 +414,7 @@ void gemm(hipStream_t st, const uint16_t *A, int64_t ars, int64_t acs, const uin
     return 0;
   };
   GemmArgs g{A, ars, acs, B, brs, bcs, C, crs, M, N, K, bias, relu, Cb, beta, nullptr, 1,
-             run_of(A, ars, acs), run_of(B, bcs, brs)};
+             run_of(A, ars, acs), run_of(B, bcs, brs), mask};
   dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
   const int tiles = (int)(grid.x * grid.y);
   constexpr int kGrain = 16;   // split-K sizing unit (slices are then rounded up to TK)
@@ -506,15 +500,12 @@ void train_step_async(qpx_mlp *m, int n, float lr, float *loss_out) {
   // layer 3: dW3 = H2^T dZ3 (H x H, K = n); db3 = colsum dZ3; dH2 = dZ3 W3^T
   gemm(st, m->H2b.p, 1, Hd, m->dZ3b.p, Hd, 1, m->gW3.p, Hd, Hd, Hd, n, nullptr, 0, nullptr, 0.f, &m->ws);
   colsum(st, m->dZ3.p, nullptr, n, Hd, m->gb3.p, m->ws);
-  gemm(st, m->dZ3b.p, Hd, 1, m->W3b.p, 1, Hd, m->dZ2.p, Hd, n, Hd, Hd, nullptr, 0, nullptr, 0.f);
-  hipLaunchKernelGGL(relu_mask_kernel, dim3(grid1((int64_t)n * Hd)), dim3(256), 0, st, m->dZ2.p, m->Z2.p,
-                     (int64_t)n * Hd, m->dZ2b.p);
+  // dZ2 = (dZ3 W3^T) masked by Z2 > 0, with its bf16 copy, in the epilogue
+  gemm(st, m->dZ3b.p, Hd, 1, m->W3b.p, 1, Hd, m->dZ2.p, Hd, n, Hd, Hd, nullptr, 0, m->dZ2b.p, 0.f, nullptr, m->Z2.p);
   // layer 2
   gemm(st, m->H1b.p, 1, Hd, m->dZ2b.p, Hd, 1, m->gW2.p, Hd, Hd, Hd, n, nullptr, 0, nullptr, 0.f, &m->ws);
   colsum(st, m->dZ2.p, nullptr, n, Hd, m->gb2.p, m->ws);
-  gemm(st, m->dZ2b.p, Hd, 1, m->W2b.p, 1, Hd, m->dZ1.p, Hd, n, Hd, Hd, nullptr, 0, nullptr, 0.f);
-  hipLaunchKernelGGL(relu_mask_kernel, dim3(grid1((int64_t)n * Hd)), dim3(256), 0, st, m->dZ1.p, m->Z1.p,
-                     (int64_t)n * Hd, m->dZ1b.p);
+  gemm(st, m->dZ2b.p, Hd, 1, m->W2b.p, 1, Hd, m->dZ1.p, Hd, n, Hd, Hd, nullptr, 0, m->dZ1b.p, 0.f, nullptr, m->Z1.p);
   // layer 1: dW1 = X^T dZ1 (D x H); dX = dZ1 W1^T (n x D)
   gemm(st, m->Xb.p, 1, D, m->dZ1b.p, Hd, 1, m->gW1.p, Hd, D, Hd, n, nullptr, 0, nullptr, 0.f, &m->ws);
   colsum(st, m->dZ1.p, nullptr, n, Hd, m->gb1.p, m->ws);
