@@ -126,6 +126,24 @@ int qpx_select_stats(qpx_ctx *ctx, int64_t *stats4);
 int qpx_select_generators(qpx_ctx *ctx, int n_states, const double *coef, int n_sel, const double *E0, int n_list,
                           const int *gens, double *gen_pt2);
 
+/* The reference's selection task farm over an explicit task list
+ * (plugins/Full_CI_ZMQ/zmq_selection.irp.f:54-100 queues "i_generator subset"
+ * tasks; run_selection_slave.irp.f:46-106 runs each with
+ * select_connected(i_generator, E0, pt2, buf, subset)).  Task t processes
+ * the blocks of generator task_gen[t] (1-based index into psi_det_sorted,
+ * <= n_sel) that pass the subset filter task_subset[t] (0 = every block,
+ * s > 0 = maskInd mod fragment_count == s-1, fragment_count = n_alpha^2,
+ * selection.irp.f:3-10,561-562); task_subset may be NULL (all 0).  A
+ * generator may appear in several tasks with distinct nonzero subsets.
+ * Ownership of blocks and bans follow the generator indices as in
+ * qpx_select, whichever tasks are listed.  pt2 (n_states) receives the sum
+ * over the tasks and out_dets / out_vals / *n_out the n_keep best external
+ * determinants of the merged buffer (as qpx_select); task_pt2
+ * (n_states, n_tasks) column-major, may be NULL, each task's EN-PT2. */
+int qpx_select_tasks(qpx_ctx *ctx, int n_states, const double *coef, int n_sel, const double *E0, int64_t n_keep,
+                     int n_tasks, const int *task_gen, const int *task_subset, double *pt2, uint64_t *out_dets,
+                     double *out_vals, int64_t *n_out, double *task_pt2);
+
 /* make_s2_eigenfunction (src/Determinants/occ_pattern.irp.f:251-309): the
  * determinants that share an occupation pattern with a determinant of
  * psi_det (N_int,2,n_det) and are not in it, for n_alpha alpha electrons.

@@ -76,7 +76,7 @@ def lib():
         L.oq_sel_buffer_destroy.argtypes = [C.c_void_p]
         L.oq_select_all.argtypes = [C.c_void_p, C.c_void_p, c_dp, c_dp, C.c_void_p, C.c_int]
         L.oq_select_tasks.argtypes = [C.c_void_p, C.c_void_p, c_dp, c_dp, C.c_void_p, C.c_int, C.c_int,
-                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int), c_dp]
         L.oq_select_connected.argtypes = [C.c_void_p, C.c_void_p, C.c_int, c_dp, c_dp, C.c_void_p, C.c_int]
         _LIB = L
     return _LIB
@@ -162,12 +162,27 @@ class Ham:
         return lib().oq_diag_H_mat_elem_fock(self.ptr, _u64(ref), _u64(pert), _dp(f))
 
 
+class _BilinearStruct(C.Structure):
+    # oq_bilinear (qp_oracle.h): 1-based int arrays as in the reference
+    _fields_ = [("n_det", C.c_int64), ("nint", C.c_int), ("n_alpha", C.c_int), ("n_beta", C.c_int),
+                ("alpha_unique", c_u64p), ("beta_unique", c_u64p),
+                ("rows", c_ip), ("columns", c_ip), ("order", c_ip), ("order_reverse", c_ip),
+                ("columns_loc", c_ip), ("transp_rows", c_ip), ("transp_columns", c_ip), ("transp_order", c_ip),
+                ("transp_rows_loc", c_ip), ("order_transp_reverse", c_ip)]
+
+
 class Bilinear:
     def __init__(self, dets):
         self.dets = _dets(dets)
         self.n_det = self.dets.shape[0]
         self.n_int = self.dets.shape[-1]
         self.ptr = lib().oq_bilinear_create(_u64(self.dets), self.n_det, self.n_int)
+
+    def order(self) -> np.ndarray:
+        """psi_bilinear_matrix_order - 1: the psi_det index (0-based) of
+        each bilinear position"""
+        b = _BilinearStruct.from_address(self.ptr)
+        return np.ctypeslib.as_array(b.order, shape=(self.n_det,)).astype(np.int64) - 1
 
     def __del__(self):
         if getattr(self, "ptr", None) and _LIB is not None:
@@ -198,6 +213,23 @@ def sigma_work_sample(ham: Ham, bil: Bilinear, nst: int, istep: int, nthreads: i
     s_t = np.zeros((n, nst))
     lib().oq_H_S2_u_0_work(ham.ptr, bil.ptr, _dp(v_t), _dp(s_t), _dp(u_t), nst, 1, n, 0, istep, nthreads)
     return len(range(1, n + 1, istep)), v_t
+
+
+def sigma_rows(ham: Ham, bil: Bilinear, u: np.ndarray, istep: int, nthreads: int = 0):
+    """Rows of v = H u and s = S^2 u for every istep-th bilinear position,
+    by the work routine on those rows only; u (N_det, N_st) in psi_det order.
+    Returns (psi indices of the rows, v rows, s rows)."""
+    u = np.asarray(u, dtype=np.float64)
+    if u.ndim == 1:
+        u = u[:, None]
+    n, nst = u.shape
+    order = bil.order()
+    u_t = np.ascontiguousarray(u[order])          # (N_det, N_st) state-fastest, bilinear order
+    v_t = np.zeros((n, nst))
+    s_t = np.zeros((n, nst))
+    lib().oq_H_S2_u_0_work(ham.ptr, bil.ptr, _dp(v_t), _dp(s_t), _dp(u_t), nst, 1, n, 0, istep, nthreads)
+    k = np.arange(0, n, istep)
+    return order[k], v_t[k], s_t[k]
 
 
 def count_pairs(ham: Ham, bil: Bilinear, nthreads: int = 0) -> int:
@@ -306,6 +338,7 @@ def selection(ham: Ham, dets_sorted, coefs_sorted, n_gen: int, n_sel: int, E0, n
         ctx.block_keep_offsets = bo.ctypes.data_as(C.POINTER(C.c_int64))
     E0 = np.ascontiguousarray(np.atleast_1d(np.asarray(E0, dtype=np.float64)))
     pt2 = np.zeros(ns)
+    task_pt2 = None
     buf = lib().oq_sel_buffer_create(n_keep, 2 * n_keep, nint)
     if tasks is None:
         lib().oq_select_all(ham.ptr, C.byref(ctx), _dp(E0), _dp(pt2), buf, nthreads)
@@ -313,16 +346,17 @@ def selection(ham: Ham, dets_sorted, coefs_sorted, n_gen: int, n_sel: int, E0, n
         tk = np.asarray(tasks, dtype=np.int32).reshape(-1, 2)
         tg = np.ascontiguousarray(tk[:, 0])
         tsb = np.ascontiguousarray(tk[:, 1])
+        task_pt2 = np.zeros((tk.shape[0], ns))
         ip = C.POINTER(C.c_int)
         lib().oq_select_tasks(ham.ptr, C.byref(ctx), _dp(E0), _dp(pt2), buf, nthreads, tk.shape[0],
-                              tg.ctypes.data_as(ip), tsb.ctypes.data_as(ip))
+                              tg.ctypes.data_as(ip), tsb.ctypes.data_as(ip), _dp(task_pt2))
     b = _SelBuf.from_address(buf)
     cur = b.cur
     out_d = np.ctypeslib.as_array(b.det, shape=(b.cap * 2 * nint,))[: cur * 2 * nint].reshape(cur, 2, nint).copy()
     out_v = np.ctypeslib.as_array(b.val, shape=(b.cap,))[:cur].copy()
     lib().oq_sel_buffer_destroy(buf)
     res = dict(pt2=pt2, dets=out_d, vals=out_v, n_candidates=ctx.n_candidates, n_gathers=ctx.n_gathers,
-               n_blocks=ctx.n_blocks, n_blocks_total=ctx.n_blocks_total)
+               n_blocks=ctx.n_blocks, n_blocks_total=ctx.n_blocks_total, task_pt2=task_pt2)
     if log is not None:
         res["block_log"] = log[: ctx.block_log_n].copy()
     return res

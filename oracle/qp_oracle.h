@@ -160,8 +160,10 @@ void oq_select_connected(const oq_ham *H, oq_sel_ctx *S, int i_generator, const 
  * per-thread buffers merged as the collector does). */
 void oq_select_all(const oq_ham *H, oq_sel_ctx *S, const double *E0, double *pt2,
                    oq_sel_buffer *out, int nthreads);
+/* At most OQ_MAXNS states in a selection. */
+#define OQ_MAXNS 8
 void oq_select_tasks(const oq_ham *H, oq_sel_ctx *S, const double *E0, double *pt2, oq_sel_buffer *out,
-                     int nthreads, int n_tasks, const int *task_gen, const int *task_subset);
+                     int nthreads, int n_tasks, const int *task_gen, const int *task_subset, double *task_pt2);
 
 #ifdef __cplusplus
 }

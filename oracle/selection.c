@@ -736,15 +736,17 @@ void oq_select_connected(const oq_ham *H, oq_sel_ctx *S, int i_generator, const 
 
 void oq_select_all(const oq_ham *H, oq_sel_ctx *S, const double *E0, double *pt2, oq_sel_buffer *out,
                    int nthreads) {
-  oq_select_tasks(H, S, E0, pt2, out, nthreads, 0, NULL, NULL);
+  oq_select_tasks(H, S, E0, pt2, out, nthreads, 0, NULL, NULL, NULL);
 }
 
 /* The task farm of ZMQ_selection (zmq_selection.irp.f:20-40): one task is a
    (subset, i_generator) pair; subset 0 = every block of the generator,
    subset k = the blocks with maskInd mod fragment_count == k-1
-   (selection.irp.f:561-562).  n_tasks == 0 runs every generator whole. */
+   (selection.irp.f:561-562).  n_tasks == 0 runs every generator whole.
+   task_pt2 (NS x n_tasks, may be NULL) receives each task's own EN-PT2, the
+   pt2 a slave sends back for one task (run_pt2_slave.irp.f:66-80). */
 void oq_select_tasks(const oq_ham *H, oq_sel_ctx *S, const double *E0, double *pt2, oq_sel_buffer *out,
-                     int nthreads, int n_tasks, const int *task_gen, const int *task_subset) {
+                     int nthreads, int n_tasks, const int *task_gen, const int *task_subset, double *task_pt2) {
   int NS = S->n_states, nint = H->nint;
   if (nthreads <= 0) nthreads = omp_get_max_threads();
   for (int st = 0; st < NS; ++st) pt2[st] = 0.0;
@@ -757,8 +759,15 @@ void oq_select_tasks(const oq_ham *H, oq_sel_ctx *S, const double *E0, double *p
     bufs[t] = oq_sel_buffer_create(N, 2 * N, nint);
     if (n_tasks > 0) {
 #pragma omp for schedule(dynamic, 1)
-      for (int k = 0; k < n_tasks; ++k)
-        oq_select_connected(H, S, task_gen[k], E0, pt2t + (int64_t)t * NS, bufs[t], task_subset[k]);
+      for (int k = 0; k < n_tasks; ++k) {
+        double tp[OQ_MAXNS];
+        for (int st = 0; st < NS; ++st) tp[st] = 0.0;
+        oq_select_connected(H, S, task_gen[k], E0, tp, bufs[t], task_subset[k]);
+        for (int st = 0; st < NS; ++st) {
+          pt2t[(int64_t)t * NS + st] += tp[st];
+          if (task_pt2) task_pt2[(int64_t)k * NS + st] = tp[st];
+        }
+      }
     } else {
 #pragma omp for schedule(dynamic, 1)
       for (int g = 1; g <= S->n_gen; ++g) oq_select_connected(H, S, g, E0, pt2t + (int64_t)t * NS, bufs[t], 0);

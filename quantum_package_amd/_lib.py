@@ -19,6 +19,8 @@ SIGNATURES = [
     ("qpx_create", C.c_int, [C.c_int, C.POINTER(c_vp)]),
     ("qpx_make_s2_eigenfunction", C.c_int, [C.c_int, C.c_int64, c_u64p, C.c_int, C.c_int64, c_u64p, c_i64p]),
     ("qpx_select_generators", C.c_int, [c_vp, C.c_int, c_dp, C.c_int, c_dp, C.c_int, c_ip, c_dp]),
+    ("qpx_select_tasks", C.c_int, [c_vp, C.c_int, c_dp, C.c_int, c_dp, C.c_int64, C.c_int, c_ip, c_ip, c_dp,
+                                   c_u64p, c_dp, c_i64p, c_dp]),
     ("qpx_destroy", C.c_int, [c_vp]),
     ("qpx_last_error", C.c_int, [c_vp, C.c_char_p, C.c_int]),
     ("qpx_set_verbosity", C.c_int, [c_vp, C.c_int]),
@@ -50,8 +52,8 @@ SIGNATURES = [
                                      C.POINTER(C.c_float)]),
 ]
 
-# internal hooks used by the tests (not part of the C-ABI header)
-EXTRA = [
+# include/qpx_debug.h: test and benchmark hooks
+DEBUG = [
     ("qpx_debug_force_sigma_path", C.c_int, [c_vp, C.c_int, c_ip]),
     ("qpx_debug_sigma_kernel_times", C.c_int, [c_vp, C.c_int, C.c_int, c_vp, c_vp, c_vp, c_vp,
                                                C.POINTER(C.c_float), C.c_int]),
@@ -71,7 +73,7 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"libqpx.so not built ({LIB_PATH}); run __graft_entry__.build()")
     L = C.CDLL(LIB_PATH)
-    for name, res, args in SIGNATURES + EXTRA:
+    for name, res, args in SIGNATURES + DEBUG:
         f = getattr(L, name)  # AttributeError: the library misses a declared entry point
         f.restype = res
         f.argtypes = args

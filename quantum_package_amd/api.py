@@ -167,6 +167,31 @@ class Context:
                                                  g.ctypes.data_as(_lib.c_ip), _dp(out)))
         return out
 
+    def select_tasks(self, coef_sorted, n_sel: int, E0, N: int, tasks):
+        """The selection task farm over explicit (i_generator, subset) tasks
+        (1-based generator indices into the current psi_det_sorted; subset 0
+        = every block).  Returns (pt2[N_states], dets[n, 2, N_int], vals[n],
+        task_pt2[n_tasks, N_states])."""
+        coef = np.asfortranarray(coef_sorted, dtype=np.float64)
+        if coef.ndim == 1:
+            coef = coef[:, None].copy(order="F")
+        ns = coef.shape[1]
+        tk = np.asarray(tasks, dtype=np.int32).reshape(-1, 2)
+        tg = np.ascontiguousarray(tk[:, 0])
+        tsb = np.ascontiguousarray(tk[:, 1])
+        E0 = np.ascontiguousarray(np.atleast_1d(E0), dtype=np.float64)
+        pt2 = np.zeros(ns)
+        out_d = np.zeros((max(N, 1), 2, self.n_int), dtype=np.uint64)
+        out_v = np.zeros(max(N, 1))
+        n_out = C.c_int64(0)
+        tpt2 = np.zeros((tk.shape[0], ns))
+        self._check(self.L.qpx_select_tasks(self.ptr, ns, _dp(coef), n_sel, _dp(E0), N, tk.shape[0],
+                                            tg.ctypes.data_as(_lib.c_ip), tsb.ctypes.data_as(_lib.c_ip), _dp(pt2),
+                                            out_d.ctypes.data_as(_lib.c_u64p), _dp(out_v), C.byref(n_out),
+                                            _dp(tpt2)))
+        n = n_out.value
+        return pt2, out_d[:n].copy(), out_v[:n].copy(), tpt2
+
     def select_blocks(self, coef_sorted, n_gen: int, n_sel: int, E0, N: int, keep=None, info: bool = True):
         """Selection restricted to the blocks with keep[b] != 0 (one byte per
         block of the first n_gen generators, in the reference's block order;

@@ -102,6 +102,10 @@ class CipsiResult:
     energies: np.ndarray
     pt2: np.ndarray
     history: list = field(default_factory=list)
+    # True when pt2 comes from a sweep over the proposed blocks only: it then
+    # lacks the contribution of the blocks left out and is a lower bound of
+    # the EN-PT2 magnitude, not the EN-PT2 itself
+    pt2_partial: bool = False
 
 
 def cipsi(ctx, H, n_det_max: int, n_states: int = 1, n_states_diag: int = 4, s2_eig: bool = True,
@@ -121,6 +125,7 @@ def cipsi(ctx, H, n_det_max: int, n_states: int = 1, n_states_diag: int = 4, s2_
     coef[0, 0] = 1.0
     history = []
     pt2 = np.full(n_states, -np.inf)
+    pt2_partial = False
     energies = None
 
     def diagonalize(dets, coef):
@@ -169,7 +174,9 @@ def cipsi(ctx, H, n_det_max: int, n_states: int = 1, n_states_diag: int = 4, s2_
         energies, coef = diagonalize(dets, coef)
         n_det = dets.shape[0]
         rec = dict(n_det=n_det, energies=energies.tolist(), pt2=None, t_diag=time.time() - t0)
-        if n_det >= n_det_max or np.max(np.abs(pt2)) <= pt2_max:
+        # the PT2 stop test needs the full EN-PT2: a proposed sweep's partial
+        # sum could only understate it
+        if n_det >= n_det_max or (not pt2_partial and np.max(np.abs(pt2)) <= pt2_max):
             history.append(rec)
             log(f"cipsi: N_det={n_det} E={energies} (stop)")
             break
@@ -187,6 +194,7 @@ def cipsi(ctx, H, n_det_max: int, n_states: int = 1, n_states_diag: int = 4, s2_
         else:
             pt2, new_dets = _proposed_selection(ctx, H, proposer, dets_s, coef_s, n_gen, n_sel, E0, to_select,
                                                 len(history), full_sweep_every, rec, log, audit_proposal)
+        pt2_partial = rec.get("sweep") == "proposed"
         rec.update(pt2=pt2.tolist(), n_gen=n_gen, n_sel=n_sel, n_new=int(new_dets.shape[0]),
                    t_sel=time.time() - t1)
         # new determinants with zero coefficients, psi normalised
@@ -203,7 +211,7 @@ def cipsi(ctx, H, n_det_max: int, n_states: int = 1, n_states_diag: int = 4, s2_
         history.append(rec)
         log(f"cipsi: N_det={n_det} E={energies} PT2={pt2} gen={n_gen} sel={n_sel} -> {dets.shape[0]} "
             f"({rec['t_diag']:.1f}s diag, {rec['t_sel']:.1f}s sel)")
-    return CipsiResult(dets=dets, coef=coef, energies=energies, pt2=pt2, history=history)
+    return CipsiResult(dets=dets, coef=coef, energies=energies, pt2=pt2, history=history, pt2_partial=pt2_partial)
 
 
 def _proposed_selection(ctx, H, proposer, dets_s, coef_s, n_gen, n_sel, E0, to_select, it, full_every, rec, log,

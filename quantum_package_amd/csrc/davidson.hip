@@ -454,7 +454,13 @@ extern "C" int qpx_davidson(qpx_ctx *c, int N_st, int N_st_diag, double threshol
     // diagonalization_hs2.irp.f:110-114
     QPX_REQUIRE((int64_t)N_st_diag * 3 <= sze, "error in Davidson: increase n_det_max_jacobi (N_st_diag*3 > sze)");
     int itermax = (int)std::max<int64_t>(3, std::min<int64_t>(sze_max, sze / N_st_diag));
-    QPX_REQUIRE(N_st_diag * itermax <= 32, "N_st_diag * davidson_sze_max must be <= 32");
+    // The subspace holds at most 32 vectors (the projection and Ritz kernels
+    // keep a row of it in registers / LDS): with more vectors per block the
+    // restart comes earlier than the reference's davidson_sze_max would put
+    // it.  32 / N_st_diag >= 4 for N_st_diag <= 8, so every restart still
+    // keeps at least three blocks of history.
+    QPX_REQUIRE(sze_max >= 1, "davidson_sze_max must be >= 1");
+    itermax = std::min(itermax, 32 / N_st_diag);
     const int nc = N_st_diag * itermax;
     const double sz = 0.5 * (c->H.na - c->H.nb);
     const double sz2 = sz * (sz - 1.0);

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdio>
 #include <vector>
 #include <hip/hip_bf16.h>
 #include "qpx_internal.h"
@@ -586,6 +587,24 @@ int mguard(qpx_mlp *m, F &&f) {
   }
 }
 
+// The descriptors index the embedding table and the one-hot slots: a row
+// outside {h1, h2 in [0, n_orb), spin class in 0..2, degree in 0..4} would
+// make the feature and embedding-gradient kernels read or write out of
+// bounds, so it is rejected on the host before anything is uploaded.
+void check_desc(const qpx_mlp *m, int64_t n, const int *desc) {
+  QPX_REQUIRE(desc, "null descriptor array");
+  for (int64_t i = 0; i < n; ++i) {
+    const int *d = desc + 4 * i;
+    if (d[0] < 0 || d[0] >= m->n_orb || d[1] < 0 || d[1] >= m->n_orb || d[2] < 0 || d[2] > 2 || d[3] < 0 ||
+        d[3] > 4) {
+      char buf[160];
+      std::snprintf(buf, sizeof buf, "descriptor row %lld = (%d, %d, %d, %d) out of range (n_orb = %d)",
+                    (long long)i, d[0], d[1], d[2], d[3], m->n_orb);
+      QPX_REQUIRE(false, buf);
+    }
+  }
+}
+
 uint64_t sm64(uint64_t &s) {
   uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
@@ -680,6 +699,8 @@ extern "C" int qpx_mlp_train_step(qpx_mlp *m, int n, const int *desc, const floa
   return mguard(m, [&] {
     QPX_HIP(hipSetDevice(m->device));
     QPX_REQUIRE(n > 0, "empty batch");
+    QPX_REQUIRE(fdesc && target, "null argument");
+    check_desc(m, n, desc);
     ensure(m, n);
     QPX_HIP(hipMemcpyAsync(m->desc.p, desc, sizeof(int) * 4 * n, hipMemcpyHostToDevice, m->st));
     QPX_HIP(hipMemcpyAsync(m->fdesc.p, fdesc, sizeof(float) * 3 * n, hipMemcpyHostToDevice, m->st));
@@ -695,6 +716,7 @@ extern "C" int qpx_mlp_train_epochs(qpx_mlp *m, int n, const int *desc, const fl
     QPX_HIP(hipSetDevice(m->device));
     QPX_REQUIRE(n > 0 && batch >= 1 && batch <= n && epochs >= 1, "need 1 <= batch <= n and epochs >= 1");
     QPX_REQUIRE(desc && fdesc && target && epoch_loss, "null argument");
+    check_desc(m, n, desc);
     ensure(m, batch);
     DBuf<int> desc_all, perm_d;
     DBuf<float> fdesc_all, tgt_all;
@@ -735,6 +757,8 @@ extern "C" int qpx_mlp_train_epochs(qpx_mlp *m, int n, const int *desc, const fl
 extern "C" int qpx_mlp_predict(qpx_mlp *m, int n, const int *desc, const float *fdesc, float *out) {
   return mguard(m, [&] {
     QPX_HIP(hipSetDevice(m->device));
+    QPX_REQUIRE(n >= 0 && (n == 0 || (fdesc && out)), "bad prediction arguments");
+    check_desc(m, n, desc);
     const int chunk = 65536;
     for (int i0 = 0; i0 < n; i0 += chunk) {
       int c = std::min(chunk, n - i0);

@@ -36,6 +36,8 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
+#include <utility>
 #include <vector>
 #include "qpx_internal.h"
 #include "qpx_device.h"
@@ -1481,11 +1483,19 @@ struct Pool {
   int64_t cap = 0;
 };
 
+// A list of the reference's selection tasks: task t enumerates the blocks of
+// generator gen[t] (1-based) whose maskInd satisfies the subset filter
+// subset[t] (0 = every block), as run_selection_slave does for one
+// "i_generator subset" task (plugins/Full_CI_ZMQ/run_selection_slave.irp.f:46-106).
+struct TaskList {
+  std::vector<int> gen, subset;
+};
+
 template <int NI>
 void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel, const double *E0, int64_t n_keep,
                    int subset, double *pt2_out, uint64_t *out_dets, double *out_vals, int64_t *n_out,
                    const uint8_t *keep_mask, double *block_pt2_out, int64_t *block_info_out,
-                   const std::vector<int> *gen_list = nullptr, double *gen_pt2_out = nullptr) {
+                   const TaskList *tasks = nullptr, double *task_pt2_out = nullptr) {
   Wfn &W = c->W;
   Ham &H = c->H;
   hipStream_t st = c->stream;
@@ -1519,14 +1529,21 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
   // all blocks in generator order
   std::vector<BlockDesc> all;
   std::vector<int> gen_first;  // first block of each generator
-  std::vector<uint8_t> listed;
-  if (gen_list) {
-    listed.assign((size_t)n_gen + 1, 0);
-    for (int g : *gen_list) listed[g] = 1;
+  std::vector<int> blk_task;   // task of each block (task lists only; -1 = none)
+  // task lists: the tasks of each generator, as (first, count) into task_of
+  std::vector<int> gt_first, gt_count, task_of;
+  if (tasks) {
+    gt_first.assign((size_t)n_gen + 2, 0);
+    gt_count.assign((size_t)n_gen + 2, 0);
+    for (int g : tasks->gen) gt_count[g]++;
+    for (int g = 1; g <= n_gen; ++g) gt_first[g + 1] = gt_first[g] + gt_count[g];
+    task_of.assign(tasks->gen.size(), 0);
+    std::vector<int> fill(gt_first);
+    for (size_t t = 0; t < tasks->gen.size(); ++t) task_of[fill[tasks->gen[t]]++] = (int)t;
   }
   for (int g = 1; g <= n_gen; ++g) {
     gen_first.push_back((int)all.size());
-    if (gen_list && !listed[g]) continue;   // only the listed generators get blocks
+    if (tasks && !gt_count[g]) continue;   // only the listed generators get blocks
     const uint64_t *gd = &psi_h[(size_t)(g - 1) * 2 * NI];
     std::vector<int> hl[2];
     for (int s = 0; s < 2; ++s)
@@ -1555,6 +1572,16 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
             b.mask_ind = maskInd;
             bool keep = (subset == 0 || (maskInd % fragment_count) == subset - 1);
             if (keep_mask) keep = keep && keep_mask[all.size()] != 0;
+            if (tasks) {
+              int owner = -1;
+              for (int q = 0; q < gt_count[g]; ++q) {
+                const int t = task_of[gt_first[g] + q];
+                const int sb = tasks->subset[t];
+                if (sb == 0 || (maskInd % fragment_count) == sb - 1) owner = t;
+              }
+              keep = keep && owner >= 0;
+              blk_task.push_back(owner);
+            }
             b.gslot = keep ? 1 : 0;  // temporary: keep flag
             all.push_back(b);
           }
@@ -1573,7 +1600,9 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
   P.count.alloc(2);
   QPX_HIP(hipMemsetAsync(P.count.p, 0, sizeof(unsigned long long) * 2, st));
   int64_t pool_n = 0;
-  double tau = 0.0;
+  // the pool keeps candidates with min-state e <= tau; nothing is kept when
+  // no determinant is asked for
+  double tau = n_keep > 0 ? 0.0 : -INFINITY;
 
   std::vector<double> blk_pt2_all((size_t)n_blocks_total * NS, 0.0);
   std::vector<int> blk_done(n_blocks_total, 0);
@@ -1792,14 +1821,15 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
     pt2_out[s] = acc;
   }
   if (block_pt2_out) std::memcpy(block_pt2_out, blk_pt2_all.data(), sizeof(double) * blk_pt2_all.size());
-  if (gen_list && gen_pt2_out) {
-    // per generator, its blocks in block order (select_connected's sum)
-    for (size_t i = 0; i < gen_list->size(); ++i) {
-      const int g = (*gen_list)[i];
+  if (tasks && task_pt2_out) {
+    // per task, its blocks in block order (select_connected's sum)
+    for (size_t t = 0; t < tasks->gen.size(); ++t) {
+      const int g = tasks->gen[t];
       for (int st = 0; st < NS; ++st) {
         double acc = 0.0;
-        for (int b = gen_first[g - 1]; b < gen_first[g]; ++b) acc += blk_pt2_all[(size_t)b * NS + st];
-        gen_pt2_out[i * NS + st] = acc;
+        for (int b = gen_first[g - 1]; b < gen_first[g]; ++b)
+          if (blk_task[b] == (int)t) acc += blk_pt2_all[(size_t)b * NS + st];
+        task_pt2_out[t * NS + st] = acc;
       }
     }
   }
@@ -1873,6 +1903,39 @@ extern "C" int qpx_debug_select_block_ticks(qpx_ctx *c, int64_t *ticks, int64_t 
   });
 }
 
+// Validated task list: generators in 1..n_sel, subsets in 0..fragment_count,
+// no generator listed twice with the same subset or with subset 0 and
+// another subset.  Returns the largest generator index.
+static int check_tasks(qpx_ctx *c, int n_sel, int n_tasks, const int *gens, const int *subsets, TaskList &T) {
+  QPX_REQUIRE(n_tasks >= 1 && gens, "need at least one task");
+  QPX_REQUIRE(n_sel >= 1 && n_sel <= c->W.n_det, "need 1 <= n_sel <= n_det");
+  const int fc = c->H.na * c->H.na;
+  T.gen.assign(gens, gens + n_tasks);
+  T.subset.assign(n_tasks, 0);
+  if (subsets) T.subset.assign(subsets, subsets + n_tasks);
+  std::vector<std::pair<int, int>> key(n_tasks);
+  for (int t = 0; t < n_tasks; ++t) {
+    QPX_REQUIRE(T.gen[t] >= 1 && T.gen[t] <= n_sel, "generator indices must be in 1..n_sel");
+    QPX_REQUIRE(T.subset[t] >= 0 && T.subset[t] <= fc, "subset must be in 0..fragment_count (n_alpha^2)");
+    key[t] = {T.gen[t], T.subset[t]};
+  }
+  std::sort(key.begin(), key.end());
+  for (int t = 1; t < n_tasks; ++t)
+    QPX_REQUIRE(key[t].first != key[t - 1].first || (key[t].second != key[t - 1].second && key[t - 1].second != 0),
+                "a generator is listed twice with the same subset, or with subset 0 and another subset");
+  return key.back().first;
+}
+
+template <typename F>
+static void dispatch_nint(int nint, F &&f) {
+  switch (nint) {
+    case 1: f(std::integral_constant<int, 1>()); break;
+    case 2: f(std::integral_constant<int, 2>()); break;
+    case 3: f(std::integral_constant<int, 3>()); break;
+    default: f(std::integral_constant<int, 4>()); break;
+  }
+}
+
 // EN-PT2 of each listed generator (select_connected's sum over its blocks)
 extern "C" int qpx_select_generators(qpx_ctx *c, int n_states, const double *coef, int n_sel, const double *E0,
                                      int n_list, const int *gens, double *gen_pt2) {
@@ -1881,26 +1944,38 @@ extern "C" int qpx_select_generators(qpx_ctx *c, int n_states, const double *coe
     QPX_REQUIRE(c->W.valid, "no wavefunction set");
     QPX_REQUIRE(n_states >= 1 && n_states <= MAXNS, "n_states must be in 1..4");
     QPX_REQUIRE(n_list >= 1 && gens && gen_pt2 && coef && E0, "bad arguments");
-    QPX_REQUIRE(n_sel >= 1 && n_sel <= c->W.n_det, "need 1 <= n_sel <= n_det");
-    std::vector<int> list(gens, gens + n_list);
-    int n_gen = 0;
-    {
-      std::vector<int> sorted(list);
-      std::sort(sorted.begin(), sorted.end());
-      QPX_REQUIRE(sorted.front() >= 1 && sorted.back() <= n_sel, "generator indices must be in 1..n_sel");
-      QPX_REQUIRE(std::adjacent_find(sorted.begin(), sorted.end()) == sorted.end(), "repeated generator index");
-      n_gen = sorted.back();
-    }
+    TaskList T;
+    const int n_gen = check_tasks(c, n_sel, n_list, gens, nullptr, T);
     double pt2[MAXNS];
     uint64_t dummy_det[2 * NINT_MAX];
     double dummy_val;
     int64_t n_out = 0;
-    switch (c->W.nint) {
-      case 1: run_selection<1>(c, n_states, coef, n_gen, n_sel, E0, 1, 0, pt2, dummy_det, &dummy_val, &n_out, nullptr, nullptr, nullptr, &list, gen_pt2); break;
-      case 2: run_selection<2>(c, n_states, coef, n_gen, n_sel, E0, 1, 0, pt2, dummy_det, &dummy_val, &n_out, nullptr, nullptr, nullptr, &list, gen_pt2); break;
-      case 3: run_selection<3>(c, n_states, coef, n_gen, n_sel, E0, 1, 0, pt2, dummy_det, &dummy_val, &n_out, nullptr, nullptr, nullptr, &list, gen_pt2); break;
-      default: run_selection<4>(c, n_states, coef, n_gen, n_sel, E0, 1, 0, pt2, dummy_det, &dummy_val, &n_out, nullptr, nullptr, nullptr, &list, gen_pt2); break;
-    }
+    dispatch_nint(c->W.nint, [&](auto ni) {
+      run_selection<decltype(ni)::value>(c, n_states, coef, n_gen, n_sel, E0, 1, 0, pt2, dummy_det, &dummy_val,
+                                         &n_out, nullptr, nullptr, nullptr, &T, gen_pt2);
+    });
+  });
+}
+
+// The reference's selection task farm over an explicit list of (generator,
+// subset) tasks, with one merged buffer (zmq_selection.irp.f:54-100,
+// run_selection_slave.irp.f:46-106)
+extern "C" int qpx_select_tasks(qpx_ctx *c, int n_states, const double *coef, int n_sel, const double *E0,
+                                int64_t n_keep, int n_tasks, const int *task_gen, const int *task_subset,
+                                double *pt2, uint64_t *out_dets, double *out_vals, int64_t *n_out,
+                                double *task_pt2) {
+  return qpx_guard(c, [&] {
+    QPX_HIP(hipSetDevice(c->device));
+    QPX_REQUIRE(c->W.valid, "no wavefunction set");
+    QPX_REQUIRE(n_states >= 1 && n_states <= MAXNS, "n_states must be in 1..4");
+    QPX_REQUIRE(coef && E0 && pt2 && n_out && n_keep >= 0, "bad selection arguments");
+    QPX_REQUIRE(n_keep == 0 || (out_dets && out_vals), "null output buffers");
+    TaskList T;
+    const int n_gen = check_tasks(c, n_sel, n_tasks, task_gen, task_subset, T);
+    dispatch_nint(c->W.nint, [&](auto ni) {
+      run_selection<decltype(ni)::value>(c, n_states, coef, n_gen, n_sel, E0, n_keep, 0, pt2, out_dets, out_vals,
+                                         n_out, nullptr, nullptr, nullptr, &T, task_pt2);
+    });
   });
 }
 

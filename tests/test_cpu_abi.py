@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("qpx.h", "qpx_proposal.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("qpx.h", "qpx_proposal.h", "qpx_debug.h")]
 LIB = os.path.join(ROOT, "quantum_package_amd", "libqpx.so")
 
 
@@ -43,8 +43,20 @@ def test_library_exports_every_declared_symbol():
 
 def test_python_mirror_binds_declared_symbols():
     from quantum_package_amd import _lib
-    bound = {n for n, _, _ in _lib.SIGNATURES}
+    bound = {n for n, _, _ in _lib.SIGNATURES + _lib.DEBUG}
     assert set(declared_functions()) <= bound
+
+
+def test_library_exports_nothing_undeclared():
+    """every qpx_* symbol the library exports is declared in include/*.h"""
+    if not os.path.exists(LIB):
+        pytest.skip("libqpx.so not built")
+    r = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.skip("nm not available")
+    exported = set(re.findall(r"\s[TW]\s+(qpx_\w+)$", r.stdout, flags=re.M))
+    assert exported, "no qpx_ symbol found"
+    assert exported <= set(declared_functions()), sorted(exported - set(declared_functions()))
 
 
 def test_create_without_gpu_fails_cleanly():

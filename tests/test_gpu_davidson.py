@@ -72,3 +72,20 @@ def test_davidson_mixed_precision(ctx):
     _say(f"  fp64 {e64[:2]} {n64}  mixed {emx[:2]} {nmx}")
     assert nmx[1] > 0
     assert np.max(np.abs(e64[:2] - emx[:2])) < 1e-6   # < 1 micro-Hartree per root
+
+
+def test_davidson_six_vectors_per_block(ctx):
+    """N_st_diag = 6 with the reference's default davidson_sze_max = 8: the
+    subspace is capped at 32 vectors (5 blocks), so the restart comes
+    earlier; the energies still match the oracle's Davidson."""
+    M, na, nb, seed = 8, 4, 4, 1001
+    H = synthetic_hamiltonian(M, na, nb, seed=seed)
+    dets = fci_space(M, na, nb)
+    _say("[davidson] fci-m8-4a4b N_st=3 N_st_diag=6")
+    ctx.set_hamiltonian(H)
+    ctx.set_wavefunction(dets)
+    g = _hf_guess(dets, M, na, nb, 3)
+    e, s2, u, ns = ctx.davidson_diag_hs2(g, N_st=3, N_st_diag=6, threshold=1e-10, sze_max=8)
+    eo, _, _, _ = O.davidson(O.Ham(H), O.Bilinear(dets), dets, g, n_st=3, n_st_diag=6, threshold=1e-10)
+    _say(f"  gpu {e[:3]} oracle {eo[:3]} sigma calls {ns}")
+    assert np.max(np.abs(e[:3] - eo[:3])) < 1e-9

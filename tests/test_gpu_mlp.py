@@ -128,3 +128,28 @@ def test_training_reduces_loss():
     _say(f"  loss {l0:.4f} -> {l1:.4f}")
     m.close()
     assert l1 < 0.1 * l0
+
+
+def test_out_of_range_descriptors_are_rejected(mlp):
+    """A descriptor row outside the embedding table or the one-hot slots is
+    an error returned before anything reaches the device."""
+    from quantum_package_amd.api import QpxError
+    rng = np.random.default_rng(2)
+    n = 64
+    desc = np.stack([rng.integers(0, 40, n), rng.integers(0, 40, n), rng.integers(0, 3, n),
+                     rng.integers(0, 5, n)], axis=1).astype(np.int32)
+    fdesc = rng.standard_normal((n, 3)).astype(np.float32)
+    y = rng.standard_normal(n).astype(np.float32)
+    _say("[mlp] out-of-range descriptors")
+    before = mlp.get("W1").copy()
+    for col, bad in ((0, 40), (1, -1), (2, 3), (3, 5)):
+        d = desc.copy()
+        d[7, col] = bad
+        with pytest.raises(QpxError):
+            mlp.train_step(d, fdesc, y, 1e-3)
+        with pytest.raises(QpxError):
+            mlp.predict(d, fdesc)
+        with pytest.raises(QpxError):
+            mlp.train_epochs(d, fdesc, y, 1e-3, 1, 16, 0)
+    assert np.array_equal(mlp.get("W1"), before)
+    assert np.isfinite(mlp.predict(desc, fdesc)).all()

@@ -312,3 +312,175 @@ def test_selection_brute_force(n_states):
     near = {tuple(d.ravel().tolist()) for v, d in vals if abs(v - cut) <= 4 * np.spacing(abs(cut))}
     assert (want ^ got) <= near
     assert res["n_candidates"] >= len(vals)
+
+
+# ------------------------------------------------- generator/selector split
+def _holes(det):
+    return [bitstring_to_list(det[0]), bitstring_to_list(det[1])]
+
+
+def _block_index(gen):
+    """maskInd of every (s1, h1, s2, h2) block of one generator, in the
+    reference's loop order (selection.irp.f:412-420,556-557): s1 = 1..2, i1
+    from the last hole down, s2 = s1..2, i2 from the last hole down to
+    i1+1 (same spin) or 1."""
+    h = _holes(gen)
+    out, m = {}, -1
+    for s1 in (1, 2):
+        for i1 in range(len(h[s1 - 1]), 0, -1):
+            for s2 in range(s1, 3):
+                ib = i1 + 1 if s1 == s2 else 1
+                for i2 in range(len(h[s2 - 1]), ib - 1, -1):
+                    m += 1
+                    out[(s1, h[s1 - 1][i1 - 1], s2, h[s2 - 1][i2 - 1])] = m
+    return out
+
+
+def _block_of(gen, D):
+    """The block of generator gen that enumerates the external determinant
+    D.  Doubles sit in the block of their two holes; single excitations are
+    folded into the alpha-beta blocks: an alpha single h -> p where the beta
+    hole is gen's last beta orbital, a beta single where the alpha hole is
+    gen's last alpha orbital (the monoAdo / monoBdo unbans, :551-559)."""
+    hg, hd = _holes(gen), _holes(D)
+    holes = [(s + 1, o) for s in range(2) for o in hg[s] if o not in hd[s]]
+    if len(holes) == 2:
+        (s1, a), (s2, b) = sorted(holes)
+        return (s1, a, s2, b)
+    (s, o), = holes
+    return (1, o, 2, hg[1][-1]) if s == 1 else (1, hg[0][-1], 2, o)
+
+
+def _brute_selection(ham, psi, coef, n_gen, n_sel, E0, M, subset=0, fragment_count=None):
+    """EN-PT2 and keys from the definition: every determinant within two
+    excitations of a generator and not in psi, counted once -- by the first
+    generator it is connected to, in the block that enumerates it there --
+    with V = sum over all selectors of c_S <D|H|S>."""
+    ns = coef.shape[1]
+    inpsi = {tuple(d.ravel().tolist()) for d in psi}
+    ext = [D for D in _first_order_space(psi[:n_gen], M) if tuple(D.ravel().tolist()) not in inpsi]
+    pt2 = np.zeros(ns)
+    vals = []
+    for D in ext:
+        g = next(i for i in range(n_gen) if ham.degree(D, psi[i]) <= 2)
+        if subset:
+            m = _block_index(psi[g])[_block_of(psi[g], D)]
+            if m % fragment_count != subset - 1:
+                continue
+        V = np.array([sum(coef[i, k] * ham.i_H_j(D, psi[i]) for i in range(n_sel)) for k in range(ns)])
+        if np.all(V == 0):
+            continue
+        dE = E0 - ham.diag(D)
+        e = 0.5 * (np.sign(dE) * np.sqrt(dE * dE + 4 * V * V) - dE)
+        pt2 += e
+        vals.append((min(e.min(), 0.0), D))
+    vals.sort(key=lambda t: t[0])
+    return pt2, vals
+
+
+def _check_top(res, vals, n_keep):
+    want = {tuple(d.ravel().tolist()) for v, d in vals[:n_keep]}
+    got = {tuple(d.ravel().tolist()) for d in res["dets"]}
+    cut = vals[min(n_keep, len(vals)) - 1][0]
+    near = {tuple(d.ravel().tolist()) for v, d in vals if abs(v - cut) <= 4 * np.spacing(abs(cut))}
+    assert (want ^ got) <= near
+
+
+def _split_psi(seed, n_states, n=16, M=8, na=3, nb=3):
+    H = synthetic_hamiltonian(M, na, nb, seed=seed)
+    ham = O.Ham(H)
+    rng = np.random.default_rng(seed)
+    hf = hf_determinant(M, na, nb)
+    psi, seen = [hf], {tuple(hf.ravel().tolist())}
+    while len(psi) < n:
+        d = _excite(rng, hf, M, int(rng.integers(1, 4)))
+        k = tuple(d.ravel().tolist())
+        if k not in seen:
+            seen.add(k)
+            psi.append(d)
+    psi = np.array(psi, dtype=np.uint64)
+    coef = rng.standard_normal((n, n_states)) * 0.1
+    coef[0, 0] = 1.0
+    if n_states > 1:
+        coef[1, 1] = 0.9
+    coef /= np.linalg.norm(coef, axis=0)
+    order = np.argsort(-(coef ** 2).mean(axis=1), kind="stable")
+    psi, coef = psi[order], coef[order]
+    Hm = np.array([[ham.i_H_j(psi[i], psi[j]) for j in range(n)] for i in range(n)])
+    E0 = np.array([coef[:, k] @ Hm @ coef[:, k] for k in range(n_states)])
+    return H, ham, psi, coef, E0
+
+
+@pytest.mark.parametrize("n_states,n_gen,n_sel", [(1, 5, 12), (2, 4, 16), (1, 9, 9)])
+def test_selection_brute_force_generator_selector_split(n_states, n_gen, n_sel):
+    """n_gen < n_sel: the generators own the external space (each external
+    determinant counted by its first connected generator), the couplings run
+    over every selector (selection.irp.f:50-79,320-403,1088-1171)."""
+    H, ham, psi, coef, E0 = _split_psi(300 + n_gen, n_states)
+    res = O.selection(ham, psi, coef, n_gen=n_gen, n_sel=n_sel, E0=E0, n_keep=20)
+    pt2, vals = _brute_selection(ham, psi, coef, n_gen, n_sel, E0, H.M)
+    assert np.allclose(res["pt2"], pt2, atol=1e-12, rtol=1e-10)
+    _check_top(res, vals, 20)
+
+
+@pytest.mark.parametrize("subset", [1, 2, 5, 9])
+def test_selection_brute_force_subset(subset):
+    """subset > 0 keeps the blocks with maskInd mod fragment_count ==
+    subset-1 (selection.irp.f:3-10,556-557); fragment_count = n_alpha^2."""
+    H, ham, psi, coef, E0 = _split_psi(400 + subset, 1)
+    n_gen, n_sel = 6, psi.shape[0]
+    fc = H.n_alpha ** 2
+    tasks = [(g, subset) for g in range(1, n_gen + 1)]
+    res = O.selection(ham, psi, coef, n_gen=n_gen, n_sel=n_sel, E0=E0, n_keep=15, tasks=tasks)
+    pt2, vals = _brute_selection(ham, psi, coef, n_gen, n_sel, E0, H.M, subset=subset, fragment_count=fc)
+    assert np.allclose(res["pt2"], pt2, atol=1e-12, rtol=1e-10)
+    if vals:
+        _check_top(res, vals, 15)
+
+
+def test_selection_subsets_partition_the_generator():
+    """The fragments of one generator partition its blocks: their PT2 sum to
+    the whole generator's (oracle task farm against itself, all subsets)."""
+    H, ham, psi, coef, E0 = _split_psi(500, 1)
+    n = psi.shape[0]
+    whole = O.selection(ham, psi, coef, n_gen=n, n_sel=n, E0=E0, n_keep=5, tasks=[(3, 0)])["pt2"]
+    parts = sum(O.selection(ham, psi, coef, n_gen=n, n_sel=n, E0=E0, n_keep=5, tasks=[(3, s)])["pt2"]
+                for s in range(1, H.n_alpha ** 2 + 1))
+    assert np.allclose(parts, whole, rtol=1e-12, atol=1e-15)
+
+
+# ------------------------------------------------------------ block network
+def test_mlp_reference_against_autograd():
+    """oracle/mlp_ref.forward_backward with the bf16 rounding switched off is
+    the exact gradient of its loss: compared with torch.autograd on the same
+    network written independently (float64)."""
+    torch = pytest.importorskip("torch")
+    from oracle import mlp_ref
+    rng = np.random.default_rng(5)
+    n, n_orb = 37, 11
+    P = {"W1": rng.standard_normal((80, 256)) * 0.1, "W2": rng.standard_normal((256, 256)) * 0.06,
+         "W3": rng.standard_normal((256, 256)) * 0.06, "b1": rng.standard_normal(256) * 0.1,
+         "b2": rng.standard_normal(256) * 0.1, "b3": rng.standard_normal(256) * 0.1,
+         "w4": rng.standard_normal(256) * 0.06, "b4": np.array([0.3]), "emb": rng.standard_normal((n_orb, 32)) * 0.3}
+    desc = np.stack([rng.integers(0, n_orb, n), rng.integers(0, n_orb, n), rng.integers(0, 3, n),
+                     rng.integers(0, 5, n)], axis=1)
+    fdesc = rng.standard_normal((n, 3))
+    target = rng.standard_normal(n)
+    y, loss, g = mlp_ref.forward_backward(P, desc, fdesc, target, round_bf16=False)
+
+    T = {k: torch.tensor(v, dtype=torch.float64, requires_grad=True) for k, v in P.items()}
+    d = torch.tensor(desc)
+    X = torch.cat([T["emb"][d[:, 0]], T["emb"][d[:, 1]], torch.tensor(fdesc),
+                   torch.nn.functional.one_hot(d[:, 2], 3).double(), torch.nn.functional.one_hot(d[:, 3], 5).double(),
+                   torch.zeros(n, 5, dtype=torch.float64)], dim=1)
+    h = torch.relu(X @ T["W1"] + T["b1"])
+    h = torch.relu(h @ T["W2"] + T["b2"])
+    h = torch.relu(h @ T["W3"] + T["b3"])
+    yt = h @ T["w4"] + T["b4"]
+    lt = ((yt - torch.tensor(target)) ** 2).mean()
+    lt.backward()
+    assert np.allclose(y, yt.detach().numpy(), rtol=1e-12, atol=1e-12)
+    assert abs(loss - lt.item()) <= 1e-12 * max(1.0, abs(loss))
+    for k in P:
+        want = T[k].grad.numpy()
+        assert np.allclose(g[k], want, rtol=1e-10, atol=1e-12), k
