@@ -6,9 +6,11 @@ determinants, N_int = 1) with 4 vectors (3 roots + 1 extra, the Davidson
 block size) in FP64, inputs resident in HBM.  value = determinants / s.
 
 The JSON line also carries:
-  roofline      algorithmic bytes of one sigma call (per-pair u gather +
-                integral, plus the per-determinant reads/writes) over the
-                measured duration of its two kernels, against 8 TB/s;
+  roofline      the dominant (column) kernel against its bound, the LDS
+                read bandwidth measured on this GPU: the alpha-beta pairs'
+                u-row and integral bytes it reads from LDS per launch over
+                its event-timed duration; with its measured HBM traffic
+                (rocprofv3 counters, calibrated) and the whole call's;
   cpu_baseline  the CPU oracle (a C/OpenMP restatement of the reference's
                 H_S2_u_0_nstates_openmp_work, the reference itself cannot be
                 built here) timed on a bounded, strided sample of the same
@@ -67,6 +69,7 @@ def main():
     ap.add_argument("--no-pt2", action="store_true")
     ap.add_argument("--pt2-ndet", type=int, default=100000)
     ap.add_argument("--pt2-seconds", type=float, default=20.0)
+    ap.add_argument("--pt2-per-stratum", type=int, default=16)
     ap.add_argument("--no-pt2-stoch", action="store_true")
     ap.add_argument("--no-selected", action="store_true")
     ap.add_argument("--sel-small", type=int, default=8000,
@@ -156,12 +159,28 @@ def main():
                                                       __import__("ctypes").c_float)), 3))
     P = fci_pairs(M, na) * n_det
     B = sigma_alg_bytes(n_det, P, nst, w, 1)
-    achieved = B / (ms * 1e-3)
-    traffic = None
+    traffic = traffic_col = None
     tpath = os.path.join(ROOT, "profiles", "sigma_dense_fci14_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_sigma")
+            tj = json.load(f)
+        traffic = tj.get("hbm_bytes_per_sigma")
+        for name, kv in tj.get("kernels", {}).items():
+            if "sigma_dense_col_kernel" in name and ("double" if prec == FP64 else "float") in name:
+                traffic_col = kv.get("hbm_bytes")
+    # Roofline of the dominant kernel, the column kernel (about 80 % of the
+    # call).  It serves the alpha-beta pairs' reads -- per pair a row of u
+    # (8 N_st w bytes) and an integral (8 w) -- from LDS, where the survey's
+    # gather model counts them as HBM gathers; its bound is the LDS read
+    # bandwidth, measured on this GPU by a conflict-free LDS read kernel.
+    lds_peak = __import__("ctypes").c_double(0.0)
+    ctx._check(ctx.L.qpx_debug_lds_bandwidth(ctx.ptr, __import__("ctypes").byref(lds_peak)))
+    lds_peak = lds_peak.value
+    n_s = na * (M - na)
+    P_ab = n_det * n_s * n_s
+    lds_bytes = P_ab * (8 * nst + 8) * w
+    t_col = float(kms[0]) * 1e-3
+    achieved = lds_bytes / t_col
 
     out = {
         "metric": METRIC,
@@ -181,9 +200,18 @@ def main():
                                "(3 roots + 1, Davidson block)",
                    "n_det": n_det, "n_states_per_call": nst, "connected_pairs": P,
                    "kernel_ms": {"sigma_dense_col": float(kms[0]), "sigma_dense_row": float(kms[1])}},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "alg_bytes_per_launch": B},
+        "roofline": {"bound": "lds", "kernel": f"sigma_dense_col_kernel ({'FP64' if prec == FP64 else 'FP32'}, "
+                                               f"{nst} vectors, {n_s} alpha singles per string)",
+                     "achieved": achieved / 1e9, "peak": lds_peak / 1e9, "unit": "GB/s", "frac": achieved / lds_peak,
+                     "traffic": traffic_col,
+                     "hbm_frac": (traffic_col / t_col / HBM_PEAK) if traffic_col else None,
+                     "alg_bytes_per_launch": lds_bytes,
+                     "note": ("achieved = alpha-beta pairs x (u row + integral) bytes read from LDS per launch / "
+                              "the column kernel's event-timed duration; peak = LDS read bandwidth measured on this "
+                              "GPU; traffic = the launch's HBM bytes (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                              "calibrated), hbm_frac = traffic / duration / 8 TB/s"),
+                     "call": {"survey_gather_model_bytes": B, "traffic": traffic,
+                              "hbm_frac": (traffic / (ms * 1e-3) / HBM_PEAK) if traffic else None}},
     }
 
     if not args.no_cpu:
@@ -608,31 +636,62 @@ def pt2_leg(args, log):
     ctx.set_wavefunction(ds)
     E0 = ctx.u_0_H_u_0(cs[:, 0])
 
-    def run(n_gen):
+    def run_tasks(gens):
         t = time.time()
-        pt2, sel, _ = ctx.ZMQ_selection(cs, n_gen, n, E0, n)
+        pt2, sel, _, _ = ctx.select_tasks(cs, n, E0, n, [(int(g), 0) for g in gens])
         return time.time() - t, pt2, ctx.select_stats(), sel.shape[0]
 
-    log("bench: pt2 calibration on 8 generators ...")
-    dt, _, st, _ = run(min(n, 8))           # warm-up and calibration
-    rate_gen = min(n, 8) / max(dt, 1e-3)
-    n_gen = int(min(n, max(8, rate_gen * args.pt2_seconds)))
-    log(f"bench: pt2 {rate_gen:.1f} generators/s; timing {n_gen} generators ...")
-    dt, pt2, st, n_new = run(n_gen)
-    res = {"value": st["candidates"] / dt, "unit": "candidates/s", "dtype": "f64", "seconds": dt,
-           "generators": n_gen, "selectors": n, "candidates": st["candidates"], "blocks": st["blocks"],
-           "selected": n_new, "pt2": pt2.tolist(),
+    log("bench: pt2 warm-up on generators 1..8 ...")
+    run_tasks(range(1, 9))
+    # Stratified sample of the whole generator range: the 64 leading
+    # generators (the heaviest, HF among them) all, then 10 strata of equal
+    # width over the rest with args.pt2_per_stratum generators drawn in each;
+    # each stratum is one selection call (to_select = N_det) timed on its own
+    # and weighted by (stratum width / drawn) to project the full sweep.
+    rng = np.random.default_rng(seed)
+    head = np.arange(1, min(n, 64) + 1)
+    edges = np.linspace(head[-1] + 1, n + 1, 11).astype(int)
+    strata = [(head, 1.0)]
+    for a, b in zip(edges[:-1], edges[1:]):
+        if b > a:
+            k = min(args.pt2_per_stratum, b - a)
+            strata.append((np.sort(rng.choice(np.arange(a, b), size=k, replace=False)), (b - a) / k))
+    rows = []
+    t_tot = c_tot = 0.0
+    t_proj = c_proj = 0.0
+    g_tot = b_tot = 0
+    for gens, wgt in strata:
+        dt, _, st, _ = run_tasks(gens)
+        rows.append({"first": int(gens[0]), "last": int(gens[-1]), "drawn": int(gens.size), "weight": float(wgt),
+                     "seconds": dt, "candidates": st["candidates"], "blocks": st["blocks"]})
+        t_tot += dt
+        c_tot += st["candidates"]
+        t_proj += wgt * dt
+        c_proj += wgt * st["candidates"]
+        g_tot += st["gathers"]
+        b_tot += st["blocks"]
+        log(f"bench: pt2 stratum {int(gens[0])}..{int(gens[-1])} ({gens.size} drawn): {dt:.2f} s, "
+            f"{st['candidates']} candidates")
+    n_drawn = int(sum(r["drawn"] for r in rows))
+    res = {"value": c_tot / t_tot, "unit": "candidates/s", "dtype": "f64", "seconds": t_tot,
+           "generators": n_drawn, "selectors": n, "candidates": int(c_tot), "blocks": b_tot,
+           "projected_full_sweep": {"seconds": t_proj, "candidates": c_proj, "candidates_per_s": c_proj / t_proj,
+                                    "generators": n},
+           "strata": rows,
            "workload": f"selection + EN-PT2, synthetic Hamiltonian seed {seed}: {M} MOs, {na} alpha + {nb} beta "
-                       f"electrons, N_int=2; psi = {n} determinants (HF + one selection step, diagonalised); the "
-                       f"{n_gen} leading generators by weight against all {n} selectors, to_select = N_det"}
+                       f"electrons, N_int=2; psi = {n} determinants (HF + one selection step, diagonalised), all of "
+                       f"them selectors, to_select = N_det; a stratified sample of {n_drawn} of the {n} generators "
+                       f"(the {head.size} leading ones, then {args.pt2_per_stratum} drawn in each of 10 equal "
+                       f"strata), one timed selection call per stratum"}
     # algorithmic bytes of the selection: 8 per integral/coefficient gather,
     # (8 N_states + 16) per evaluated candidate, 8 N_states M^2 per block tile
-    b_alg = 8 * st["gathers"] + st["candidates"] * (8 * 1 + 16) + st["blocks"] * 8 * 1 * M * M
-    res["roofline"] = {"bound": "hbm", "achieved": b_alg / dt / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                       "frac": b_alg / dt / HBM_PEAK, "alg_bytes": b_alg,
-                       "note": "whole qpx_select call (host block enumeration and final sort included)"}
-    log(f"bench: pt2 {n_gen} generators: {dt:.2f} s, {st['candidates']} candidates, "
-        f"{res['value']:.3e} candidates/s")
+    b_alg = 8 * g_tot + c_tot * (8 * 1 + 16) + b_tot * 8 * 1 * M * M
+    res["roofline"] = {"bound": "hbm", "achieved": b_alg / t_tot / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                       "frac": b_alg / t_tot / HBM_PEAK, "alg_bytes": b_alg,
+                       "note": "whole qpx_select_tasks calls of the sample (host block enumeration and final "
+                               "sort included)"}
+    log(f"bench: pt2 sample of {n_drawn} generators: {t_tot:.2f} s, {res['value']:.3e} candidates/s; projected "
+        f"full sweep {t_proj:.1f} s")
     if not args.no_pt2_stoch:
         # the full EN-PT2 of this psi (every determinant a generator) is out of
         # reach exactly; the hybrid stochastic estimate, capped in time
@@ -664,7 +723,7 @@ def pt2_leg(args, log):
 
         k = 2 * args.cpu_threads
         dtc, r = cpu(k)
-        k2 = int(min(n_gen * nfrag, k * args.cpu_seconds / max(dtc, 1e-3)))
+        k2 = int(min(64 * nfrag, k * args.cpu_seconds / max(dtc, 1e-3)))
         if k2 > k:
             k = k2
             dtc, r = cpu(k)

@@ -30,6 +30,12 @@ int qpx_debug_force_sigma_path(qpx_ctx *ctx, int path, int *last_path);
 int qpx_debug_sigma_kernel_times(qpx_ctx *ctx, int prec, int n_st, const void *u_t, void *v_t, void *s_t,
                                  void *stream, float *ms, int reps);
 
+/* LDS read bandwidth of the context's device (bytes/s): two 1024-thread
+ * workgroups per CU read 16 bytes per lane per iteration from LDS in the
+ * conflict-free pattern, timed with HIP events (the peak the dense-map
+ * sigma kernel's LDS roofline is quoted against). */
+int qpx_debug_lds_bandwidth(qpx_ctx *ctx, double *bytes_per_s);
+
 /* Entries the sparse sigma walk visits per call in each bucket class (sum
  * of squared bucket sizes): visited (5) = ab, a1, a2, b1, b2.  Builds the
  * buckets when they do not exist. */

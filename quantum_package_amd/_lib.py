@@ -58,6 +58,7 @@ DEBUG = [
     ("qpx_debug_sigma_kernel_times", C.c_int, [c_vp, C.c_int, C.c_int, c_vp, c_vp, c_vp, c_vp,
                                                C.POINTER(C.c_float), C.c_int]),
     ("qpx_debug_sparse_stats", C.c_int, [c_vp, c_i64p]),
+    ("qpx_debug_lds_bandwidth", C.c_int, [c_vp, c_dp]),
     ("qpx_debug_select_block_ticks", C.c_int, [c_vp, c_i64p, C.c_int64]),
     ("qpx_debug_mlp_gemm", C.c_int, [c_vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                      C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]),

@@ -398,6 +398,7 @@ struct SelArgs {
   int64_t *blk_ncand;
   int64_t *blk_ngath;
   int64_t *blk_ticks;  // duration of the workgroup in 100 MHz wall-clock ticks (cost model of the proposal)
+  double *mat_g;       // coupling matrices in HBM (one NS*M*M slot per workgroup), when they do not fit LDS
 };
 
 template <int NI>
@@ -979,28 +980,57 @@ __device__ inline double diag_fock(const Tabs &T, const double *f, int sp, int h
 #undef JJ
 }
 
-// One selector in the LDS queue of a block: its index, kind (0: four
-// particles outside the mask, couplings precomputed; 1: three; 2: two or
-// fewer) with the number of precomputed couplings, the couplings' cells
-// (i | j << 8, two per word) and values, its coefficients and determinant.
-template <int NI>
-struct SelQ {
-  int idx;
-  int kind_n;
-  uint32_t cellw[3];
-  double hij[6];
-  double coef[MAXNS];
-  uint64_t det[2 * NI];
+// The LDS queue of a block's surviving selectors, as arrays (qcap entries):
+// per entry its index, kind (0: four particles outside the mask, couplings
+// precomputed; 1: three; 2: two or fewer) with the number of precomputed
+// couplings, the couplings' cells (i | j << 8, two per word) and values, and
+// its coefficients.  The replay of a flush keeps, per owner thread, the list
+// of (entry, coupling) records of the four-particle entries that fall on
+// its cells (rec, cursor), and the positions of the other entries (k12).
+struct QueueLds {
+  double *hij;      // 6 * qcap
+  double *coef;     // NS * qcap
+  int *idx;         // qcap
+  int *kind_n;      // qcap
+  uint32_t *cellw;  // 3 * qcap
+  int *cursor;      // SB
+  uint16_t *rec;    // 6 * qcap
+  uint16_t *k12;    // qcap
+  __host__ __device__ static size_t bytes(int qcap, int NS) {
+    return (size_t)qcap * (8 * (6 + NS) + 4 * 5 + 2 * 7) + 4 * (size_t)SB;
+  }
+  __device__ void carve(unsigned char *base, int qcap, int NS) {
+    hij = reinterpret_cast<double *>(base);
+    coef = hij + 6 * (size_t)qcap;
+    idx = reinterpret_cast<int *>(coef + (size_t)NS * qcap);
+    kind_n = idx + qcap;
+    cellw = reinterpret_cast<uint32_t *>(kind_n + qcap);
+    cursor = reinterpret_cast<int *>(cellw + 3 * (size_t)qcap);
+    rec = reinterpret_cast<uint16_t *>(cursor + SB);
+    k12 = rec + 6 * (size_t)qcap;
+  }
 };
 
-template <int NI>
+// owner thread of cell (i, j): OW * band(i) + (i + j - 2) mod OW
+__device__ inline int owner_of(int i, int j, int M) { return ((i + j - 2) & (OW - 1)) + OW * band_of(i, M); }
+
+// GM = false: the coupling matrix lives in LDS.  GM = true (N_states * M^2
+// doubles do not fit next to the queue): it lives in HBM, one slot of
+// A.mat_g per workgroup of the launch; each cell is still read and written
+// by its owner thread only.
+template <int NI, bool GM>
 __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int M = T.M, NS = A.NS;
   const int nbits = (M * M + 31) / 32;
   BlockLds L;
-  L.mat = reinterpret_cast<double *>(lds_raw);
-  L.fock = L.mat + (size_t)NS * M * M;
+  if constexpr (GM) {
+    L.mat = A.mat_g + (size_t)blockIdx.x * NS * M * M;
+    L.fock = reinterpret_cast<double *>(lds_raw);
+  } else {
+    L.mat = reinterpret_cast<double *>(lds_raw);
+    L.fock = L.mat + (size_t)NS * M * M;
+  }
   L.ban1 = reinterpret_cast<uint32_t *>(L.fock + 2 * (M + 1));
   L.ban2 = L.ban1 + nbits;
   L.borb = L.ban2 + nbits;
@@ -1203,91 +1233,119 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
   // get_d1 / get_d0 with every thread.  Each cell thus receives its terms in
   // selector order, as in the reference.
   Ctx<NI> C(T, A, L, mask);
-  SelQ<NI> *Q = reinterpret_cast<SelQ<NI> *>(red);
+  QueueLds Q;
+  Q.carve(reinterpret_cast<unsigned char *>(red), A.qcap, NS);
+  const int lane = tid & 63, wv = tid >> 6;
+  // Replay of the queue (entries 0..nq-1, in near-list order).  Each cell
+  // of the coupling matrix is touched only by its owner thread, so each
+  // thread can apply its own terms in queue order without synchronisation:
+  //   1. every thread counts the records of its entries per owner and the
+  //      positions of the three- and fewer-particle entries are compacted in
+  //      order (k12);
+  //   2. an exclusive scan of the counts gives each owner a range of rec,
+  //      and the records are scattered into it;
+  //   3. each thread sorts its records by (entry, coupling) -- the values
+  //      are distinct, so the order is fixed -- and walks the k12 list:
+  //      before entry k it adds its records of earlier entries, then runs
+  //      get_d1 / get_d0 of entry k on its cells; the records after the
+  //      last such entry follow.
+  auto cell_of = [&](int e, int c) -> uint32_t {
+    return (Q.cellw[3 * e + (c >> 1)] >> (16 * (c & 1))) & 0xffffu;
+  };
   auto flush = [&](int nq) {
-    int k = 0;
-    while (k < nq) {
-      // a run of four-particle entries: headers 8 at a time, then each
-      // thread adds the values of the cells it owns
-      const int m = min(8, nq - k);
-      int kn[8];
-      uint32_t cw[8][3];
-#pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        if (b < m) {
-          kn[b] = Q[k + b].kind_n;
-#pragma unroll
-          for (int w = 0; w < 3; ++w) cw[b][w] = Q[k + b].cellw[w];
-        } else {
-          kn[b] = 1;
-        }
-      }
-      int r = 0;
-#pragma unroll
-      for (int b = 0; b < 8; ++b)
-        if (r == b && (kn[b] & 0xf) == 0) ++r;
-      // this thread's cells among the run, in order; the first two are
-      // handled with their LDS loads issued together
-      int own_n = 0, e0 = 0, e1 = 0;
-      uint32_t c0 = 0, c1 = 0;
-#pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        if (b >= r) continue;
-        const int nc = kn[b] >> 4;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-          if (c >= nc) continue;
-          const uint32_t wd = (c < 2) ? cw[b][0] : ((c < 4) ? cw[b][1] : cw[b][2]);
-          const uint32_t cell = (wd >> (16 * (c & 1))) & 0xffffu;
-          if (!owns(cell & 0xff, cell >> 8, M)) continue;
-          const int ec = (b << 3) | c;
-          if (own_n == 0) { e0 = ec; c0 = cell; }
-          else if (own_n == 1) { e1 = ec; c1 = cell; }
-          ++own_n;
-        }
-      }
-      if (own_n == 1) {
-        C.add(c0 & 0xff, c0 >> 8, Q[k + (e0 >> 3)].coef, Q[k + (e0 >> 3)].hij[e0 & 7]);
-      } else if (own_n == 2 && c0 != c1) {
-        const SelQ<NI> &q0 = Q[k + (e0 >> 3)], &q1 = Q[k + (e1 >> 3)];
-        const double h0 = q0.hij[e0 & 7], h1 = q1.hij[e1 & 7];
-        double *m0 = L.mat + (int64_t)NS * (((int)(c0 & 0xff) - 1) + M * ((int)(c0 >> 8) - 1));
-        double *m1 = L.mat + (int64_t)NS * (((int)(c1 & 0xff) - 1) + M * ((int)(c1 >> 8) - 1));
-        for (int st = 0; st < NS; ++st) {
-          const double a0 = m0[st], a1 = m1[st], f0 = q0.coef[st], f1 = q1.coef[st];
-          m0[st] = a0 + f0 * h0;
-          m1[st] = a1 + f1 * h1;
-        }
-      } else if (own_n >= 2) {
-        // same cell twice, or more than two cells: in order, one by one
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-          if (b >= r) continue;
-          const int nc = kn[b] >> 4;
+    Q.cursor[tid] = 0;
+    __syncthreads();
+    int n12 = 0;
+    for (int e0 = 0; e0 < nq; e0 += SB) {
+      const int e = e0 + tid;
+      bool is12 = false;
+      if (e < nq) {
+        const int kn = Q.kind_n[e];
+        if ((kn & 0xf) == 0) {
+          const int nc = kn >> 4;
           for (int c = 0; c < nc; ++c) {
-            const uint32_t wd = (c < 2) ? cw[b][0] : ((c < 4) ? cw[b][1] : cw[b][2]);
-            const uint32_t cell = (wd >> (16 * (c & 1))) & 0xffffu;
-            const int ci = cell & 0xff, cj = cell >> 8;
-            if (owns(ci, cj, M)) C.add(ci, cj, Q[k + b].coef, Q[k + b].hij[c]);
+            const uint32_t cell = cell_of(e, c);
+            atomicAdd(&Q.cursor[owner_of((int)(cell & 0xff), (int)(cell >> 8), M)], 1);
           }
+        } else {
+          is12 = true;
         }
       }
-      k += r;
-      if (r < m) {
-        const SelQ<NI> &q = Q[k];
-        const int kind = q.kind_n & 0xf;
-        uint64_t d[2 * NI];
+      const unsigned long long bal = __ballot(is12);
+      const int before = __popcll(bal & ((1ULL << lane) - 1ULL));
+      if (lane == 0) wsum[wv] = __popcll(bal);
+      __syncthreads();
+      int off = 0, tot = 0;
+      for (int w = 0; w < SB / 64; ++w) {
+        if (w < wv) off += wsum[w];
+        tot += wsum[w];
+      }
+      if (is12) Q.k12[n12 + off + before] = (uint16_t)e;
+      n12 += tot;
+      __syncthreads();
+    }
+    // exclusive scan of the per-owner counts
+    const int my_cnt = Q.cursor[tid];
+    int incl = my_cnt;
 #pragma unroll
-        for (int w = 0; w < 2 * NI; ++w) d[w] = q.det[w];
-        int p[2][4], np[2], h[2][2], nh[2];
-        sel_lists<NI>(mask, d, p, np, h, nh);
-        if (kind == 1) get_d1<NI>(C, d, h, p, nh, np, sp, q.coef, A.na, A.nb);
-        else get_d0<NI>(C, d, p, sp, q.coef, A.na, A.nb);
-        ++k;
+    for (int d = 1; d < 64; d <<= 1) {
+      const int y = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += y;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int my_start = incl - my_cnt;
+    for (int w = 0; w < wv; ++w) my_start += wsum[w];
+    Q.cursor[tid] = my_start;
+    __syncthreads();
+    for (int e = tid; e < nq; e += SB) {
+      const int kn = Q.kind_n[e];
+      if ((kn & 0xf) != 0) continue;
+      const int nc = kn >> 4;
+      for (int c = 0; c < nc; ++c) {
+        const uint32_t cell = cell_of(e, c);
+        const int slot = atomicAdd(&Q.cursor[owner_of((int)(cell & 0xff), (int)(cell >> 8), M)], 1);
+        Q.rec[slot] = (uint16_t)((e << 3) | c);
       }
     }
+    __syncthreads();
+    uint16_t *mine = Q.rec + my_start;
+    for (int a = 1; a < my_cnt; ++a) {   // insertion sort (a few records per thread)
+      const uint16_t x = mine[a];
+      int b = a - 1;
+      while (b >= 0 && mine[b] > x) {
+        mine[b + 1] = mine[b];
+        --b;
+      }
+      mine[b + 1] = x;
+    }
+    int r = 0;
+    auto apply_until = [&](int k) {
+      while (r < my_cnt) {
+        const uint16_t x = mine[r];
+        const int e = x >> 3;
+        if (e >= k) break;
+        const int c = x & 7;
+        const uint32_t cell = cell_of(e, c);
+        C.add((int)(cell & 0xff), (int)(cell >> 8), Q.coef + (size_t)NS * e, Q.hij[6 * e + c]);
+        ++r;
+      }
+    };
+    for (int q = 0; q < n12; ++q) {
+      const int k = Q.k12[q];
+      apply_until(k);
+      const int kind = Q.kind_n[k] & 0xf;
+      uint64_t d[2 * NI];
+      const uint64_t *dp = A.psi + (int64_t)(Q.idx[k] - 1) * 2 * NI;
+#pragma unroll
+      for (int w = 0; w < 2 * NI; ++w) d[w] = dp[w];
+      int p[2][4], np[2], h[2][2], nh[2];
+      sel_lists<NI>(mask, d, p, np, h, nh);
+      if (kind == 1) get_d1<NI>(C, d, h, p, nh, np, sp, Q.coef + (size_t)NS * k, A.na, A.nb);
+      else get_d0<NI>(C, d, p, sp, Q.coef + (size_t)NS * k, A.na, A.nb);
+    }
+    apply_until(nq);
   };
-  const int lane = tid & 63, wv = tid >> 6;
   int qn = 0;
   // the near-list entry and determinant of the next chunk are loaded while
   // the current one is processed
@@ -1337,19 +1395,17 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
       if (pass && pos >= done && pos < done + space) {
         // the survivor fits in the queue: fill its entry (the four-particle
         // couplings are computed straight into it)
-        SelQ<NI> &q = Q[qn + pos - done];
-        q.idx = idx;
-        for (int st = 0; st < NS; ++st) q.coef[st] = A.coef[(idx - 1) + A.n_det * st];
+        const int e = qn + pos - done;
+        Q.idx[e] = idx;
+        for (int st = 0; st < NS; ++st) Q.coef[(size_t)NS * e + st] = A.coef[(idx - 1) + A.n_det * st];
         C.gath += NS;
-#pragma unroll
-        for (int w = 0; w < 2 * NI; ++w) q.det[w] = d[w];
-        D2List dl{q.cellw, q.hij, 0};
+        D2List dl{Q.cellw + 3 * e, Q.hij + 6 * e, 0};
         if (nt == 4) {
           int p[2][4], np[2], h[2][2], nh[2];
           sel_lists<NI>(mask, d, p, np, h, nh);
           get_d2<NI>(C, d, h, p, nh, np, sp, dl);
         }
-        q.kind_n = (nt == 4 ? 0 : (nt == 3 ? 1 : 2)) | (dl.n << 4);
+        Q.kind_n[e] = (nt == 4 ? 0 : (nt == 3 ? 1 : 2)) | (dl.n << 4);
       }
       const int adv = min(cnt - done, space);
       qn += adv;
@@ -1505,16 +1561,31 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
   const int nbits = (M * M + 31) / 32;
   // mat | fock | ban1, ban2, bannedOrb | flags, wavefront counts | reduction
   // scratch shared with the selector queue
-  const size_t lds_base = sizeof(double) * ((size_t)NS * M * M + 2 * (M + 1)) + sizeof(uint32_t) * (2 * nbits + 16) +
-                          sizeof(int) * 12 + 8;
+  const size_t mat_bytes = sizeof(double) * (size_t)NS * M * M;
   const size_t red_bytes = sizeof(double) * SB * std::max(NS, 1);
-  QPX_REQUIRE(lds_base + red_bytes <= (size_t)c->lds_per_block,
-              "selection: N_states * mo_num^2 coupling matrix does not fit in LDS of one workgroup");
-  const int qcap = (int)std::min<size_t>(1024, ((size_t)c->lds_per_block - lds_base) / sizeof(SelQ<NI>));
+  const size_t rest_bytes = sizeof(double) * 2 * (M + 1) + sizeof(uint32_t) * (2 * nbits + 16) + sizeof(int) * 12 + 8;
+  // the coupling matrix goes to HBM when it leaves no room for a queue of
+  // 64 entries in LDS (the reference keeps mat(N_states, M, M) on the heap
+  // for any size, selection.irp.f:409)
+  const bool gm = mat_bytes + rest_bytes + std::max(red_bytes, QueueLds::bytes(64, NS)) > (size_t)c->lds_per_block;
+  const size_t lds_base = rest_bytes + (gm ? 0 : mat_bytes);
+  QPX_REQUIRE(lds_base + red_bytes <= (size_t)c->lds_per_block, "selection: mo_num too large for the LDS layout");
+  int qcap = 0;
+  if ((size_t)c->lds_per_block > lds_base + QueueLds::bytes(0, NS))
+    qcap = (int)std::min<size_t>(1024, ((size_t)c->lds_per_block - lds_base - QueueLds::bytes(0, NS)) /
+                                           (QueueLds::bytes(1, NS) - QueueLds::bytes(0, NS)));
   QPX_REQUIRE(qcap >= 16, "selection: no LDS left for the selector queue");
-  const size_t lds = lds_base + std::max(red_bytes, (size_t)qcap * sizeof(SelQ<NI>));
-  auto kfun = select_block_kernel<NI>;
+  const size_t lds = lds_base + std::max(red_bytes, QueueLds::bytes(qcap, NS));
+  auto kfun = gm ? select_block_kernel<NI, true> : select_block_kernel<NI, false>;
   QPX_HIP(hipFuncSetAttribute((const void *)kfun, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  // HBM coupling matrices: one slot per workgroup of a launch, launches of
+  // at most mat_slots workgroups (about 1 GiB of slots)
+  DBuf<double> mat_g;
+  int64_t mat_slots = 0;
+  if (gm) {
+    mat_slots = std::max<int64_t>(256, std::min<int64_t>(4096, ((int64_t)1 << 30) / (int64_t)mat_bytes));
+    mat_g.alloc((size_t)mat_slots * NS * M * M);
+  }
 
   DBuf<double> coef_d;
   coef_d.upload(coef, (size_t)n_det * NS, st);
@@ -1737,8 +1808,28 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
       std::vector<int64_t> run_gid(nb);
       for (int i = 0; i < nb; ++i) run_gid[i] = bl_gid[pending[i]];
       int64_t pool_before = pool_n;
-      hipLaunchKernelGGL(kfun, dim3(nb), dim3(SB), lds, st, SA, T);
-      QPX_HIP(hipGetLastError());
+      if (!gm) {
+        hipLaunchKernelGGL(kfun, dim3(nb), dim3(SB), lds, st, SA, T);
+        QPX_HIP(hipGetLastError());
+      } else {
+        // launches of at most mat_slots workgroups, each on its own slot;
+        // the pool's block ids stay run-local through blk_base
+        SA.mat_g = mat_g.p;
+        for (int off = 0; off < nb; off += (int)mat_slots) {
+          const int nl = (int)std::min<int64_t>(mat_slots, nb - off);
+          SelArgs S2 = SA;
+          S2.blocks = SA.blocks + off;
+          S2.n_blocks = nl;
+          S2.blk_pt2 = SA.blk_pt2 + (size_t)off * NS;
+          S2.blk_status = SA.blk_status + off;
+          S2.blk_ncand = SA.blk_ncand + off;
+          S2.blk_ngath = SA.blk_ngath + off;
+          S2.blk_ticks = SA.blk_ticks + off;
+          S2.blk_base = off;
+          hipLaunchKernelGGL(kfun, dim3(nl), dim3(SB), lds, st, S2, T);
+          QPX_HIP(hipGetLastError());
+        }
+      }
       std::vector<double> bp((size_t)nb * NS);
       std::vector<int> bs(nb);
       std::vector<int64_t> bc(nb), bg(nb), bt(nb);

@@ -792,6 +792,53 @@ extern "C" int qpx_sigma_device(qpx_ctx *c, int prec, int nst, const void *u_t, 
   });
 }
 
+// LDS read bandwidth of the device: every lane of 1024-thread workgroups
+// (two per CU, 64 KiB of LDS each) reads 16 bytes per iteration from LDS, in
+// the conflict-free pattern (consecutive lanes, consecutive 16-byte words),
+// and sums them so that no read can be dropped.
+__global__ __launch_bounds__(1024) void lds_read_kernel(int iters, float *out) {
+  __shared__ float4 buf[4096];
+  for (int i = threadIdx.x; i < 4096; i += blockDim.x) buf[i] = make_float4(i, i + 1, i + 2, i + 3);
+  __syncthreads();
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+  const int base = threadIdx.x;
+  for (int it = 0; it < iters; ++it) {
+    const float4 x = buf[(base + 64 * it) & 4095];
+    a0 += x.x;
+    a1 += x.y;
+    a2 += x.z;
+    a3 += x.w;
+  }
+  const float r = a0 + a1 + a2 + a3;
+  if (r == -1.f) out[blockIdx.x] = r;   // never true: keeps the reads
+}
+
+extern "C" int qpx_debug_lds_bandwidth(qpx_ctx *c, double *bytes_per_s) {
+  return qpx_guard(c, [&] {
+    QPX_REQUIRE(bytes_per_s, "null output");
+    QPX_HIP(hipSetDevice(c->device));
+    int cus = 0;
+    QPX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    const int grid = 2 * cus, iters = 16384;
+    DBuf<float> out;
+    out.alloc(grid);
+    hipEvent_t e0, e1;
+    QPX_HIP(hipEventCreate(&e0));
+    QPX_HIP(hipEventCreate(&e1));
+    hipLaunchKernelGGL(lds_read_kernel, dim3(grid), dim3(1024), 0, c->stream, iters, out.p);   // warm-up
+    QPX_HIP(hipEventRecord(e0, c->stream));
+    hipLaunchKernelGGL(lds_read_kernel, dim3(grid), dim3(1024), 0, c->stream, iters, out.p);
+    QPX_HIP(hipEventRecord(e1, c->stream));
+    QPX_HIP(hipGetLastError());
+    QPX_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    QPX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *bytes_per_s = (double)grid * 1024.0 * iters * 16.0 / (ms * 1e-3);
+  });
+}
+
 // test hook: 0 = automatic, 1 = dense-map only, 2 = generic only
 extern "C" int qpx_debug_force_sigma_path(qpx_ctx *c, int path, int *last_path) {
   return qpx_guard(c, [&] {

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <numeric>
 #include <cstring>
+#include <hipcub/hipcub.hpp>
 #include "qpx_internal.h"
 #include "qpx_device.h"
 
@@ -206,6 +207,140 @@ static void build_conn(qpx_ctx *c, const DBuf<uint64_t> &str, int n, int nint, S
 
 namespace qpx {
 
+// ---------------------------------------------------------------------------
+// set-up of a wavefunction on the device: unique spin strings (stable sort of
+// the determinants by their string, then run heads), the CSC order by (beta
+// id, alpha id) and the CSR order by (alpha id, beta id), all with hipCUB
+// radix sorts.  The unique strings come in the reference's order: by
+// spin_det_search_key, the exclusive-or of the words compared as unsigned
+// (src/Determinants/spindeterminants.irp.f:9-24,74-77), and among equal keys
+// (possible for N_int > 1) by the words, first word first.
+// ---------------------------------------------------------------------------
+}  // namespace qpx
+
+namespace {
+
+// electron counts and orbital range of every determinant: bad[0] = 1 when
+// one has the wrong number of alpha / beta electrons or an orbital beyond M
+__global__ void check_dets_kernel(const uint64_t *__restrict__ psi, int64_t n, int nint, int M, int na, int nb,
+                                  int *__restrict__ bad) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    int ca = 0, cb = 0;
+    bool out = false;
+    for (int w = 0; w < nint; ++w) {
+      const uint64_t a = psi[(2 * k) * nint + w], b = psi[(2 * k + 1) * nint + w];
+      ca += __popcll(a);
+      cb += __popcll(b);
+      if (w == nint - 1 && (M & 63)) {
+        const uint64_t hi = ~((1ULL << (M & 63)) - 1);
+        out = out || (a & hi) || (b & hi);
+      }
+    }
+    if (ca != na || cb != nb || out) atomicOr(bad, 1);
+  }
+}
+
+__global__ void iota_kernel(int *__restrict__ x, int64_t n) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    x[k] = (int)k;
+}
+
+// key[t] = word w of the spin string of determinant ord[t]
+__global__ void spin_word_kernel(const uint64_t *__restrict__ psi, int64_t n, int nint, int spin, int w,
+                                 const int *__restrict__ ord, uint64_t *__restrict__ key) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    key[t] = psi[(2 * (int64_t)ord[t] + spin) * nint + w];
+}
+
+// key[t] = spin_det_search_key of the spin string of determinant ord[t]: the
+// exclusive-or of its words
+__global__ void spin_xor_kernel(const uint64_t *__restrict__ psi, int64_t n, int nint, int spin,
+                                const int *__restrict__ ord, uint64_t *__restrict__ key) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t *x = psi + (2 * (int64_t)ord[t] + spin) * nint;
+    uint64_t k = x[0];
+    for (int w = 1; w < nint; ++w) k ^= x[w];
+    key[t] = k;
+  }
+}
+
+// head[t] = 1 where the string of ord[t] differs from that of ord[t-1]
+__global__ void run_head_kernel(const uint64_t *__restrict__ psi, int64_t n, int nint, int spin,
+                                const int *__restrict__ ord, int *__restrict__ head) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    int h = 1;
+    if (t > 0) {
+      const uint64_t *a = psi + (2 * (int64_t)ord[t] + spin) * nint, *b = psi + (2 * (int64_t)ord[t - 1] + spin) * nint;
+      h = 0;
+      for (int w = 0; w < nint; ++w) h |= (a[w] != b[w]);
+    }
+    head[t] = h;
+  }
+}
+
+// id_of_det[ord[t]] = sid[t] - 1; the string of each run head goes to
+// ustr[sid[t] - 1]
+__global__ void unique_scatter_kernel(const uint64_t *__restrict__ psi, int64_t n, int nint, int spin,
+                                      const int *__restrict__ ord, const int *__restrict__ head,
+                                      const int *__restrict__ sid, int *__restrict__ id_of_det,
+                                      uint64_t *__restrict__ ustr) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const int u = sid[t] - 1;
+    id_of_det[ord[t]] = u;
+    if (head[t])
+      for (int w = 0; w < nint; ++w) ustr[(int64_t)u * nint + w] = psi[(2 * (int64_t)ord[t] + spin) * nint + w];
+  }
+}
+
+__global__ void csc_key_kernel(const int *__restrict__ aid, const int *__restrict__ bid, int64_t n, int64_t n_a,
+                               uint64_t *__restrict__ key) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    key[k] = (uint64_t)bid[k] * (uint64_t)n_a + (uint64_t)aid[k];
+}
+
+// bilinear (CSC) position t: its alpha / beta ids, the psi index, the
+// duplicate test against position t-1 and the column / row counts
+__global__ void csc_fill_kernel(const uint64_t *__restrict__ key, const int *__restrict__ perm, int64_t n,
+                                const int *__restrict__ aid, const int *__restrict__ bid, int *__restrict__ csc_a,
+                                int *__restrict__ csc_b, int64_t *__restrict__ perm64, int *__restrict__ col_cnt,
+                                int *__restrict__ row_cnt, int *__restrict__ dup) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const int k = perm[t];
+    const int a = aid[k], b = bid[k];
+    csc_a[t] = a;
+    csc_b[t] = b;
+    perm64[t] = k;
+    atomicAdd(&col_cnt[b], 1);
+    atomicAdd(&row_cnt[a], 1);
+    if (t > 0 && key[t] == key[t - 1]) atomicOr(dup, 1);
+  }
+}
+
+__global__ void csr_key_kernel(const int *__restrict__ csc_a, const int *__restrict__ csc_b, int64_t n, int64_t n_b,
+                               uint64_t *__restrict__ key) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    key[t] = (uint64_t)csc_a[t] * (uint64_t)n_b + (uint64_t)csc_b[t];
+}
+
+__global__ void csr_fill_kernel(const int *__restrict__ ord2, const int *__restrict__ csc_b, int64_t n,
+                                int *__restrict__ csr_b, int *__restrict__ csr2csc) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const int q = ord2[t];
+    csr_b[t] = csc_b[q];
+    csr2csc[t] = q;
+  }
+}
+
+int bits_for(uint64_t x) {   // bits needed to represent the values 0..x
+  int b = 1;
+  while (b < 64 && (x >> b) != 0) ++b;
+  return b;
+}
+
+}  // namespace
+
+namespace qpx {
+
 void build_wavefunction(qpx_ctx *c, int nint, int64_t n_det, const uint64_t *psi) {
   QPX_REQUIRE(c->ham_set, "qpx_set_hamiltonian must be called before qpx_set_wavefunction");
   QPX_REQUIRE(nint == c->H.nint, "n_int does not match mo_num");
@@ -214,82 +349,117 @@ void build_wavefunction(qpx_ctx *c, int nint, int64_t n_det, const uint64_t *psi
   Wfn &W = c->W;
   W.valid = false;
   W.sp.clear();
+  W.perm.clear();
   W.n_det = n_det;
   W.nint = nint;
   hipStream_t s = c->stream;
   const int M = c->H.M;
-  // electron counts
-  for (int64_t k = 0; k < n_det; ++k) {
-    int na = 0, nb = 0;
-    for (int w = 0; w < nint; ++w) {
-      na += __builtin_popcountll(psi[(2 * k) * nint + w]);
-      nb += __builtin_popcountll(psi[(2 * k + 1) * nint + w]);
-      if (w == nint - 1 && M % 64) {
-        uint64_t hi = ~((1ULL << (M % 64)) - 1);
-        QPX_REQUIRE(!(psi[(2 * k) * nint + w] & hi) && !(psi[(2 * k + 1) * nint + w] & hi),
-                    "determinant occupies an orbital beyond mo_num");
-      }
-    }
-    QPX_REQUIRE(na == c->H.na && nb == c->H.nb, "determinant electron count differs from n_alpha/n_beta");
+  const int64_t n = n_det;
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, 65536);
+  W.psi.upload(psi, (size_t)n * 2 * nint, s);
+  DBuf<int> flags;
+  flags.alloc(2);
+  QPX_HIP(hipMemsetAsync(flags.p, 0, sizeof(int) * 2, s));
+  hipLaunchKernelGGL(check_dets_kernel, dim3(grid), dim3(256), 0, s, W.psi.p, n, nint, M, c->H.na, c->H.nb, flags.p);
+  QPX_HIP(hipGetLastError());
+  // sort buffers: keys and int values, double-buffered, plus hipCUB's
+  // temporary storage sized for the largest call
+  DBuf<uint64_t> k1, k2;
+  DBuf<int> v1, v2, head, sid, aid, bid;
+  k1.alloc(n); k2.alloc(n); v1.alloc(n); v2.alloc(n); head.alloc(n); sid.alloc(n); aid.alloc(n); bid.alloc(n);
+  size_t tmp_sort = 0, tmp_scan = 0;
+  QPX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_sort, k1.p, k2.p, v1.p, v2.p, (int)n, 0, 64, s));
+  QPX_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp_scan, head.p, sid.p, (int)n, s));
+  const size_t n_cnt = (size_t)n + 1;   // the column / row counts are at most n_det + 1 entries
+  size_t tmp_scan2 = 0;
+  {
+    int *dummy = nullptr;
+    QPX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_scan2, dummy, dummy, (int)n_cnt, s));
   }
-  // unique strings per spin (lexicographic on words, high word first)
-  auto unique_spin = [&](int spin, std::vector<uint64_t> &out, std::vector<int> &idx_of_det) {
-    std::vector<int64_t> ord(n_det);
-    std::iota(ord.begin(), ord.end(), 0);
-    auto str = [&](int64_t k) { return psi + (2 * k + spin) * nint; };
-    std::sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) {
-      const uint64_t *a = str(x), *b = str(y);
-      for (int w = nint - 1; w >= 0; --w)
-        if (a[w] != b[w]) return a[w] < b[w];
-      return x < y;
-    });
-    out.clear();
-    idx_of_det.assign(n_det, -1);
-    int n = -1;
-    const uint64_t *prev = nullptr;
-    for (int64_t t = 0; t < n_det; ++t) {
-      const uint64_t *a = str(ord[t]);
-      if (!prev || std::memcmp(a, prev, sizeof(uint64_t) * nint) != 0) {
-        ++n;
-        out.insert(out.end(), a, a + nint);
-        prev = a;
-      }
-      idx_of_det[ord[t]] = n;
+  DBuf<unsigned char> temp;
+  temp.alloc(std::max({tmp_sort, tmp_scan, tmp_scan2, (size_t)1}));
+  auto unique_spin = [&](int spin, DBuf<uint64_t> &ustr, DBuf<int> &id_of_det) -> int {
+    // stable LSD passes, least significant key first: the words (last word
+    // first) when N_int > 1, then the search key (ties by psi index)
+    hipLaunchKernelGGL(iota_kernel, dim3(grid), dim3(256), 0, s, v1.p, n);
+    for (int w = (nint > 1 ? nint - 1 : -1); w >= -1; --w) {
+      if (w >= 0)
+        hipLaunchKernelGGL(spin_word_kernel, dim3(grid), dim3(256), 0, s, W.psi.p, n, nint, spin, w, v1.p, k1.p);
+      else
+        hipLaunchKernelGGL(spin_xor_kernel, dim3(grid), dim3(256), 0, s, W.psi.p, n, nint, spin, v1.p, k1.p);
+      size_t tb = temp.n;
+      QPX_HIP(hipcub::DeviceRadixSort::SortPairs(temp.p, tb, k1.p, k2.p, v1.p, v2.p, (int)n, 0, 64, s));
+      std::swap(v1.p, v2.p);
     }
-    return n + 1;
+    hipLaunchKernelGGL(run_head_kernel, dim3(grid), dim3(256), 0, s, W.psi.p, n, nint, spin, v1.p, head.p);
+    size_t tb = temp.n;
+    QPX_HIP(hipcub::DeviceScan::InclusiveSum(temp.p, tb, head.p, sid.p, (int)n, s));
+    int n_u = 0;
+    QPX_HIP(hipMemcpyAsync(&n_u, sid.p + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    QPX_HIP(hipStreamSynchronize(s));
+    ustr.alloc((size_t)n_u * nint);
+    hipLaunchKernelGGL(unique_scatter_kernel, dim3(grid), dim3(256), 0, s, W.psi.p, n, nint, spin, v1.p, head.p,
+                       sid.p, id_of_det.p, ustr.p);
+    QPX_HIP(hipGetLastError());
+    return n_u;
   };
-  std::vector<int> aid, bid;
-  W.n_a = unique_spin(0, W.astr_h, aid);
-  W.n_b = unique_spin(1, W.bstr_h, bid);
-  // CSC order: (beta, alpha); duplicates of a determinant are rejected
-  std::vector<int64_t> key(n_det);
-  W.perm.resize(n_det);
-  std::iota(W.perm.begin(), W.perm.end(), 0);
-  for (int64_t k = 0; k < n_det; ++k) key[k] = (int64_t)bid[k] * W.n_a + aid[k];
-  std::sort(W.perm.begin(), W.perm.end(), [&](int64_t x, int64_t y) { return key[x] < key[y]; });
-  std::vector<int> csc_a(n_det), csc_b(n_det), col_ptr(W.n_b + 1, 0);
-  for (int64_t t = 0; t < n_det; ++t) {
-    int64_t k = W.perm[t];
-    if (t > 0) QPX_REQUIRE(key[k] != key[W.perm[t - 1]], "duplicate determinant in psi_det");
-    csc_a[t] = aid[k];
-    csc_b[t] = bid[k];
-    col_ptr[bid[k] + 1]++;
+  int bad[2] = {0, 0};
+  QPX_HIP(hipMemcpyAsync(bad, flags.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  QPX_HIP(hipStreamSynchronize(s));
+  QPX_REQUIRE(!bad[0], "determinant electron count differs from n_alpha/n_beta, or an orbital beyond mo_num");
+  W.n_a = unique_spin(0, W.astr, aid);
+  W.n_b = unique_spin(1, W.bstr, bid);
+  // CSC order: by (beta id, alpha id); duplicates of a determinant are rejected
+  hipLaunchKernelGGL(csc_key_kernel, dim3(grid), dim3(256), 0, s, aid.p, bid.p, n, (int64_t)W.n_a, k1.p);
+  hipLaunchKernelGGL(iota_kernel, dim3(grid), dim3(256), 0, s, v1.p, n);
+  {
+    size_t tb = temp.n;
+    const int nbits = bits_for((uint64_t)W.n_a * (uint64_t)W.n_b);
+    QPX_HIP(hipcub::DeviceRadixSort::SortPairs(temp.p, tb, k1.p, k2.p, v1.p, v2.p, (int)n, 0, nbits, s));
   }
-  for (int j = 0; j < W.n_b; ++j) col_ptr[j + 1] += col_ptr[j];
-  // CSR order: (alpha, beta), pointing back to CSC positions
-  std::vector<int64_t> ord2(n_det);
-  std::iota(ord2.begin(), ord2.end(), 0);
-  std::sort(ord2.begin(), ord2.end(), [&](int64_t x, int64_t y) {
-    if (csc_a[x] != csc_a[y]) return csc_a[x] < csc_a[y];
-    return csc_b[x] < csc_b[y];
-  });
-  std::vector<int> csr_b(n_det), csr2csc(n_det), row_ptr(W.n_a + 1, 0);
-  for (int64_t t = 0; t < n_det; ++t) {
-    csr_b[t] = csc_b[ord2[t]];
-    csr2csc[t] = (int)ord2[t];
-    row_ptr[csc_a[ord2[t]] + 1]++;
+  W.csc_a.alloc(n);
+  W.csc_b.alloc(n);
+  W.perm_d.alloc(n);
+  DBuf<int> col_cnt, row_cnt;
+  col_cnt.alloc((size_t)W.n_b + 1);
+  row_cnt.alloc((size_t)W.n_a + 1);
+  QPX_HIP(hipMemsetAsync(col_cnt.p, 0, sizeof(int) * ((size_t)W.n_b + 1), s));
+  QPX_HIP(hipMemsetAsync(row_cnt.p, 0, sizeof(int) * ((size_t)W.n_a + 1), s));
+  hipLaunchKernelGGL(csc_fill_kernel, dim3(grid), dim3(256), 0, s, k2.p, v2.p, n, aid.p, bid.p, W.csc_a.p, W.csc_b.p,
+                     W.perm_d.p, col_cnt.p, row_cnt.p, flags.p + 1);
+  QPX_HIP(hipGetLastError());
+  W.col_ptr.alloc((size_t)W.n_b + 1);
+  W.row_ptr.alloc((size_t)W.n_a + 1);
+  {
+    size_t tb = temp.n;
+    QPX_HIP(hipcub::DeviceScan::ExclusiveSum(temp.p, tb, col_cnt.p, W.col_ptr.p, W.n_b + 1, s));
+    tb = temp.n;
+    QPX_HIP(hipcub::DeviceScan::ExclusiveSum(temp.p, tb, row_cnt.p, W.row_ptr.p, W.n_a + 1, s));
   }
-  for (int i = 0; i < W.n_a; ++i) row_ptr[i + 1] += row_ptr[i];
+  // CSR order: by (alpha id, beta id), pointing back to CSC positions
+  hipLaunchKernelGGL(csr_key_kernel, dim3(grid), dim3(256), 0, s, W.csc_a.p, W.csc_b.p, n, (int64_t)W.n_b, k1.p);
+  hipLaunchKernelGGL(iota_kernel, dim3(grid), dim3(256), 0, s, v1.p, n);
+  {
+    size_t tb = temp.n;
+    const int nbits = bits_for((uint64_t)W.n_a * (uint64_t)W.n_b);
+    QPX_HIP(hipcub::DeviceRadixSort::SortPairs(temp.p, tb, k1.p, k2.p, v1.p, v2.p, (int)n, 0, nbits, s));
+  }
+  W.csr_b.alloc(n);
+  W.csr2csc.alloc(n);
+  hipLaunchKernelGGL(csr_fill_kernel, dim3(grid), dim3(256), 0, s, v2.p, W.csc_b.p, n, W.csr_b.p, W.csr2csc.p);
+  QPX_HIP(hipGetLastError());
+  // host copies: the unique strings (sub-string tables of the sparse path)
+  // and the column / row lengths
+  std::vector<int> col_ptr(W.n_b + 1), row_ptr(W.n_a + 1);
+  W.astr_h.resize((size_t)W.n_a * nint);
+  W.bstr_h.resize((size_t)W.n_b * nint);
+  QPX_HIP(hipMemcpyAsync(W.astr_h.data(), W.astr.p, sizeof(uint64_t) * W.astr_h.size(), hipMemcpyDeviceToHost, s));
+  QPX_HIP(hipMemcpyAsync(W.bstr_h.data(), W.bstr.p, sizeof(uint64_t) * W.bstr_h.size(), hipMemcpyDeviceToHost, s));
+  QPX_HIP(hipMemcpyAsync(col_ptr.data(), W.col_ptr.p, sizeof(int) * col_ptr.size(), hipMemcpyDeviceToHost, s));
+  QPX_HIP(hipMemcpyAsync(row_ptr.data(), W.row_ptr.p, sizeof(int) * row_ptr.size(), hipMemcpyDeviceToHost, s));
+  QPX_HIP(hipMemcpyAsync(bad + 1, flags.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+  QPX_HIP(hipStreamSynchronize(s));
+  QPX_REQUIRE(!bad[1], "duplicate determinant in psi_det");
   W.col_len_h.resize(W.n_b);
   W.row_len_h.resize(W.n_a);
   W.max_col = W.max_row = 0;
@@ -301,23 +471,12 @@ void build_wavefunction(qpx_ctx *c, int nint, int64_t n_det, const uint64_t *psi
     W.row_len_h[i] = row_ptr[i + 1] - row_ptr[i];
     W.max_row = std::max(W.max_row, W.row_len_h[i]);
   }
-  W.psi.upload(psi, (size_t)n_det * 2 * nint, s);
-  W.astr.upload(W.astr_h, s);
-  W.bstr.upload(W.bstr_h, s);
-  W.col_ptr.upload(col_ptr, s);
-  W.csc_a.upload(csc_a, s);
-  W.csc_b.upload(csc_b, s);
-  W.row_ptr.upload(row_ptr, s);
-  W.csr_b.upload(csr_b, s);
-  W.csr2csc.upload(csr2csc, s);
-  W.perm_d.upload(W.perm, s);
   // the spin-string connectivity (all pairs of unique strings) is built on
   // first use: the sparse sigma path and the selection do not need it
   W.conn_valid = false;
   W.hdiag.alloc(n_det);
   W.sdiag.alloc(n_det);
   Ham &H = c->H;
-  int grid = (int)std::min<int64_t>((n_det + 255) / 256, 65536);
 #define QPX_DIAG(NI)                                                                                         \
   hipLaunchKernelGGL(diag_kernel<NI>, dim3(grid), dim3(256), 0, s, W.astr.p, W.bstr.p, W.csc_a.p, W.csc_b.p, \
                      n_det, M, H.hcore.p, H.Jd.p, H.Kd.p, W.hdiag.p, W.sdiag.p)
@@ -364,7 +523,10 @@ extern "C" int qpx_set_wavefunction(qpx_ctx *c, int n_int, int64_t n_det, const 
 extern "C" int qpx_get_bilinear_order(qpx_ctx *c, int64_t *perm) {
   return qpx_guard(c, [&] {
     QPX_REQUIRE(c->W.valid, "no wavefunction set");
-    std::memcpy(perm, c->W.perm.data(), sizeof(int64_t) * c->W.n_det);
+    QPX_REQUIRE(perm, "null output");
+    QPX_HIP(hipSetDevice(c->device));
+    QPX_HIP(hipMemcpyAsync(perm, c->W.perm_d.p, sizeof(int64_t) * c->W.n_det, hipMemcpyDeviceToHost, c->stream));
+    QPX_HIP(hipStreamSynchronize(c->stream));
   });
 }
 

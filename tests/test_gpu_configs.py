@@ -261,3 +261,28 @@ def test_select_tasks_rejects_bad_lists(ctx, selected128):
     for bad in ([(0, 0)], [(n + 1, 0)], [(5, 0), (5, 3)], [(5, 2), (5, 2)], [(5, 145)]):
         with pytest.raises(QpxError):
             ctx.select_tasks(cs, n, E0, 10, bad)
+
+
+def test_selection128_two_states_hbm_matrix(ctx, selected128):
+    """Two states at 128 orbitals: N_states * M^2 doubles (256 KiB) exceed
+    the LDS of a workgroup, so the coupling matrices live in HBM (the
+    reference allocates mat(N_states, M, M) on the heap for any size,
+    selection.irp.f:409).  Same bit-exact bar as one state."""
+    H, ds, cs, E0 = selected128
+    n = ds.shape[0]
+    rng = np.random.default_rng(8)
+    c2 = rng.standard_normal(n) * np.abs(cs[:, 0]) + 1e-3 * rng.standard_normal(n)
+    c2 -= (c2 @ cs[:, 0]) * cs[:, 0]
+    coef = np.asfortranarray(np.column_stack([cs[:, 0], c2 / np.linalg.norm(c2)]))
+    E0_2 = np.array([E0[0], E0[0] + 0.5])
+    tasks = [(1, 0), (2, 0), (7, 0), (40000, 0), (99000, 0)]
+    _say(f"[config selection128] two states, tasks {tasks}")
+    ctx.set_hamiltonian(H)
+    ctx.set_wavefunction(ds)
+    pt2, sel, vals, tpt2 = ctx.select_tasks(coef, n, E0_2, 20000, tasks)
+    ref = O.selection(O.Ham(H), ds, coef, 99000, n, E0_2, 20000, nthreads=NTHR, tasks=tasks)
+    _say(f"  gpu pt2 {pt2} oracle {ref['pt2']}")
+    assert np.allclose(pt2, ref["pt2"], rtol=1e-12, atol=0)
+    assert np.allclose(tpt2, ref["task_pt2"], rtol=1e-12, atol=1e-15)
+    assert np.array_equal(vals, ref["vals"])
+    assert np.array_equal(sel, ref["dets"])
