@@ -50,6 +50,15 @@ int qpx_mlp_train_step(qpx_mlp *mlp, int n, const int *desc, const float *fdesc,
                        float *loss);
 int qpx_mlp_predict(qpx_mlp *mlp, int n, const int *desc, const float *fdesc, float *out);
 
+/* qpx_mlp_predict without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The descriptors are
+ * staged and the activations kept in a file-scope device array of the
+ * library (about 10 MB per GPU, held until the process ends); n must be in
+ * 1..2048.  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (several host threads or networks) race. */
+int qpx_mlp_predict_static(qpx_mlp *mlp, int n, const int *desc, const float *fdesc, float *out);
+
 /* `epochs` epochs of Adam steps over a training set of n samples (same
  * layouts as qpx_mlp_train_step) uploaded once: each epoch orders the
  * samples by a permutation drawn from a SplitMix64 stream seeded with seed

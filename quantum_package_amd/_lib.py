@@ -45,6 +45,7 @@ SIGNATURES = [
     ("qpx_mlp_train_step", C.c_int, [c_vp, C.c_int, c_ip, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
                                      C.POINTER(C.c_float)]),
     ("qpx_mlp_predict", C.c_int, [c_vp, C.c_int, c_ip, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("qpx_mlp_predict_static", C.c_int, [c_vp, C.c_int, c_ip, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("qpx_mlp_train_epochs", C.c_int, [c_vp, C.c_int, c_ip, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
                                        C.c_int, C.c_int, C.c_uint64, C.POINTER(C.c_float)]),
     ("qpx_mlp_param", C.c_int, [c_vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int64]),

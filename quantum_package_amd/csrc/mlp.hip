@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstring>
 #include <cstdio>
+#include <mutex>
 #include <vector>
 #include <hip/hip_bf16.h>
 #include "qpx_internal.h"
@@ -377,6 +378,43 @@ __global__ void adam_multi_kernel(AdamSet S, float lr, float b1, float b2, float
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Scratch of the allocation-free prediction (qpx_mlp_predict_static): the
+// staged descriptors, the features and the three layers' activations of at
+// most kStaticRows blocks, in one file-scope device array.  Each GPU the
+// library runs on holds its own copy until the process ends; its device
+// address is looked up once per device (scratch_address) and only that
+// address is handed to kernels and copies.
+namespace {
+constexpr int kStaticRows = 2048;
+constexpr int kDin = 80, kHid = 256;
+constexpr size_t al256(size_t x) { return (x + 255) & ~size_t(255); }
+constexpr size_t kScratchBytes =
+    al256(sizeof(int) * 4 * kStaticRows) + al256(sizeof(float) * 3 * kStaticRows) +
+    al256(sizeof(float) * kDin * kStaticRows) + al256(sizeof(uint16_t) * kDin * kStaticRows) +
+    3 * al256(sizeof(float) * kHid * kStaticRows) + 3 * al256(sizeof(uint16_t) * kHid * kStaticRows) +
+    al256(sizeof(float) * kStaticRows);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kMaxDevices] = {};
+
+// device address of this device's scratch (the current device must be
+// `device`); looked up once per device
+void *scratch_address(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  QPX_REQUIRE(device >= 0 && device < kMaxDevices, "device index out of range");
+  if (!g_scratch_dev[device]) {
+    void *p = nullptr;
+    QPX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    QPX_REQUIRE(p != nullptr, "no device address for the prediction scratch");
+    g_scratch_dev[device] = p;
+  }
+  return g_scratch_dev[device];
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
 struct qpx_mlp {
   int din = 80, hid = 256, n_orb = 0;
   hipStream_t st = nullptr;
@@ -395,6 +433,7 @@ struct qpx_mlp {
   DBuf<uint16_t> Xb, H1b, H2b, H3b, dZ3b, dZ2b, dZ1b;
   DBuf<int> desc;
   DBuf<float> fdesc, tgt;
+  void *scratch = nullptr;   // device address of this device's prediction scratch
   std::string err;
 };
 
@@ -472,15 +511,52 @@ void ensure(qpx_mlp *m, int n) {
 
 int grid1(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 16384); }
 
-void forward(qpx_mlp *m, int n) {
+// the buffers of one forward pass over n rows
+struct FwdBufs {
+  const int *desc;
+  const float *fdesc;
+  float *X, *Z1, *Z2, *Z3, *y;
+  uint16_t *Xb, *H1b, *H2b, *H3b;
+};
+
+void forward_bufs(qpx_mlp *m, int n, const FwdBufs &b) {
   hipStream_t st = m->st;
   const int D = m->din, Hd = m->hid;
-  hipLaunchKernelGGL(features_kernel, dim3(grid1((int64_t)n * D)), dim3(256), 0, st, m->desc.p, m->fdesc.p, m->emb.p,
-                     n, D, m->X.p, m->Xb.p);
+  hipLaunchKernelGGL(features_kernel, dim3(grid1((int64_t)n * D)), dim3(256), 0, st, b.desc, b.fdesc, m->emb.p, n, D,
+                     b.X, b.Xb);
   // Z = act(X W + b): A = X (n x D, row-major), B = W (D x H, row-major)
-  gemm(st, m->Xb.p, D, 1, m->W1b.p, Hd, 1, m->Z1.p, Hd, n, Hd, D, m->b1.p, 1, m->H1b.p, 0.f);
-  gemm(st, m->H1b.p, Hd, 1, m->W2b.p, Hd, 1, m->Z2.p, Hd, n, Hd, Hd, m->b2.p, 1, m->H2b.p, 0.f);
-  gemm(st, m->H2b.p, Hd, 1, m->W3b.p, Hd, 1, m->Z3.p, Hd, n, Hd, Hd, m->b3.p, 1, m->H3b.p, 0.f);
+  gemm(st, b.Xb, D, 1, m->W1b.p, Hd, 1, b.Z1, Hd, n, Hd, D, m->b1.p, 1, b.H1b, 0.f);
+  gemm(st, b.H1b, Hd, 1, m->W2b.p, Hd, 1, b.Z2, Hd, n, Hd, Hd, m->b2.p, 1, b.H2b, 0.f);
+  gemm(st, b.H2b, Hd, 1, m->W3b.p, Hd, 1, b.Z3, Hd, n, Hd, Hd, m->b3.p, 1, b.H3b, 0.f);
+}
+
+void forward(qpx_mlp *m, int n) {
+  forward_bufs(m, n, FwdBufs{m->desc.p, m->fdesc.p, m->X.p, m->Z1.p, m->Z2.p, m->Z3.p, m->y.p, m->Xb.p, m->H1b.p,
+                             m->H2b.p, m->H3b.p});
+}
+
+// the forward buffers carved out of the prediction scratch at its device
+// address base (the layout kScratchBytes is sized for)
+FwdBufs static_bufs(void *base) {
+  unsigned char *p = static_cast<unsigned char *>(base);
+  auto take = [&](size_t bytes) {
+    unsigned char *q = p;
+    p += al256(bytes);
+    return q;
+  };
+  FwdBufs b{};
+  b.desc = reinterpret_cast<const int *>(take(sizeof(int) * 4 * kStaticRows));
+  b.fdesc = reinterpret_cast<const float *>(take(sizeof(float) * 3 * kStaticRows));
+  b.X = reinterpret_cast<float *>(take(sizeof(float) * kDin * kStaticRows));
+  b.Xb = reinterpret_cast<uint16_t *>(take(sizeof(uint16_t) * kDin * kStaticRows));
+  b.Z1 = reinterpret_cast<float *>(take(sizeof(float) * kHid * kStaticRows));
+  b.Z2 = reinterpret_cast<float *>(take(sizeof(float) * kHid * kStaticRows));
+  b.Z3 = reinterpret_cast<float *>(take(sizeof(float) * kHid * kStaticRows));
+  b.H1b = reinterpret_cast<uint16_t *>(take(sizeof(uint16_t) * kHid * kStaticRows));
+  b.H2b = reinterpret_cast<uint16_t *>(take(sizeof(uint16_t) * kHid * kStaticRows));
+  b.H3b = reinterpret_cast<uint16_t *>(take(sizeof(uint16_t) * kHid * kStaticRows));
+  b.y = reinterpret_cast<float *>(take(sizeof(float) * kStaticRows));
+  return b;
 }
 
 // One Adam step on the batch in place, enqueued on the network's stream
@@ -625,6 +701,8 @@ extern "C" int qpx_mlp_create(int device, int n_orb, uint64_t seed, qpx_mlp **ou
     QPX_HIP(hipSetDevice(device));
     m->device = device;
     m->n_orb = n_orb;
+    QPX_REQUIRE(m->din == kDin && m->hid == kHid, "network shape differs from the prediction scratch layout");
+    m->scratch = scratch_address(device);
     QPX_HIP(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
     const int D = m->din, Hd = m->hid;
     uint64_t s = seed;
@@ -772,6 +850,29 @@ extern "C" int qpx_mlp_predict(qpx_mlp *m, int n, const int *desc, const float *
       QPX_HIP(hipGetLastError());
       QPX_HIP(hipMemcpyAsync(out + i0, m->y.p, sizeof(float) * c, hipMemcpyDeviceToHost, m->st));
     }
+    QPX_HIP(hipStreamSynchronize(m->st));
+  });
+}
+
+// Prediction without any allocation: the descriptors are staged, and the
+// activations kept, in this device's file-scope scratch (kStaticRows rows at
+// most).  Every call on a GPU shares that scratch: calls that overlap on one
+// GPU (from several threads or networks) race.
+extern "C" int qpx_mlp_predict_static(qpx_mlp *m, int n, const int *desc, const float *fdesc, float *out) {
+  return mguard(m, [&] {
+    QPX_REQUIRE(n >= 1 && n <= kStaticRows, "n must be in 1..2048 (the rows the static scratch holds)");
+    QPX_REQUIRE(fdesc && out, "null argument");
+    check_desc(m, n, desc);
+    QPX_HIP(hipSetDevice(m->device));
+    QPX_REQUIRE(m->scratch, "no prediction scratch on this device");
+    const FwdBufs b = static_bufs(m->scratch);
+    QPX_HIP(hipMemcpyAsync(const_cast<int *>(b.desc), desc, sizeof(int) * 4 * n, hipMemcpyHostToDevice, m->st));
+    QPX_HIP(hipMemcpyAsync(const_cast<float *>(b.fdesc), fdesc, sizeof(float) * 3 * n, hipMemcpyHostToDevice, m->st));
+    forward_bufs(m, n, b);
+    hipLaunchKernelGGL(head_kernel, dim3(head_blocks(n)), dim3(256), 0, m->st, b.Z3, m->w4.p, m->b4.p, n, m->hid,
+                       (const float *)nullptr, b.y, (float *)nullptr, (float *)nullptr);
+    QPX_HIP(hipGetLastError());
+    QPX_HIP(hipMemcpyAsync(out, b.y, sizeof(float) * n, hipMemcpyDeviceToHost, m->st));
     QPX_HIP(hipStreamSynchronize(m->st));
   });
 }
