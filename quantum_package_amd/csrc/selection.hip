@@ -84,11 +84,70 @@ __device__ inline double phase_bi(const uint64_t *sel, int s1, int s2, int h1, i
   return (np & 1) ? -1.0 : 1.0;
 }
 
+// Prefix-parity masks of a selector (the reference's phasemask,
+// get_mask_phase, selection.irp.f:24-47): bit o-1 of spin s's words is the
+// parity of the occupied orbitals <= o, so pm(s, o) is one bit test.
+template <int NI>
+__device__ inline void prefix_parity(const uint64_t *sel, uint64_t *pmk) {
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    uint64_t carry = 0;   // all ones when the lower words hold an odd count
+#pragma unroll
+    for (int w = 0; w < NI; ++w) {
+      const uint64_t x = sel[s * NI + w];
+      uint64_t y = x;
+      y ^= y << 1;
+      y ^= y << 2;
+      y ^= y << 4;
+      y ^= y << 8;
+      y ^= y << 16;
+      y ^= y << 32;
+      pmk[s * NI + w] = y ^ carry;
+      carry ^= (uint64_t)0 - (uint64_t)(__popcll(x) & 1);
+    }
+  }
+}
+
+__device__ inline int pm_bit(const uint64_t *pmk_s, int o) { return (int)((pmk_s[(o - 1) >> 6] >> ((o - 1) & 63)) & 1); }
+
+// phase_bi from the prefix-parity masks
+template <int NI>
+__device__ inline double phase_pm(const uint64_t *pmk, int s1, int s2, int h1, int p1, int h2, int p2) {
+  const uint64_t *a = pmk + (s1 - 1) * NI, *b = pmk + (s2 - 1) * NI;
+  int np = pm_bit(a, h1) + pm_bit(a, p1) + pm_bit(b, h2) + pm_bit(b, p2);
+  if (p1 < h1) np++;
+  if (p2 < h2) np++;
+  if (s1 == s2 && max(h1, p1) > min(h2, p2)) np++;
+  return (np & 1) ? -1.0 : 1.0;
+}
+
 struct Tabs {
   const double *V, *hcore, *J, *JA, *fock_cs;
   int M, npair;
   double ref_energy;
+  // M^4 table, G4[((a*M + b)*M + y)*M + x] = (ax|by) (0-based), or null
+  // above its orbital limit: the lines of cells get_d1 / get_d0 fill read
+  // it along their varying orbital, so a wavefront's lanes read neighbouring
+  // addresses (the values are those of V)
+  const double *G4;
 };
+
+// Offset of <ij|kl> = (ik|jl) (1-based orbitals) in the table the line
+// kernels read: in G4 with argument `vary` (0..3 = i, j, k, l) in the
+// contiguous slot, by the 8-fold symmetry (ik|jl) = (ki|jl) = (jl|ik) =
+// (lj|ik); in V (pair matrix) when there is no G4.
+__device__ inline int64_t line_off(const Tabs &T, int i, int j, int k, int l, int vary) {
+  if (!T.G4) return pair_index(i - 1, k - 1) * (int64_t)T.npair + pair_index(j - 1, l - 1);
+  int a, x, b, y;   // (a x | b y)
+  switch (vary) {
+    case 0: a = k; x = i; b = j; y = l; break;
+    case 1: a = l; x = j; b = i; y = k; break;
+    case 2: a = i; x = k; b = j; y = l; break;
+    default: a = j; x = l; b = i; y = k; break;
+  }
+  const int M = T.M;
+  return (((int64_t)(a - 1) * M + (b - 1)) * M + (y - 1)) * M + (x - 1);
+}
 
 __device__ inline double I4(const Tabs &T, int i, int j, int k, int l) {  // <ij|kl>, 1-based
   return T.V[pair_index(i - 1, k - 1) * T.npair + pair_index(j - 1, l - 1)];
@@ -433,25 +492,31 @@ __device__ inline void borb_set(uint32_t *b, int o, int s) {
   atomicOr(&b[(s - 1) * 8 + ((o - 1) >> 5)], 1u << ((o - 1) & 31));
 }
 
-// owner of mat cell (i,j) (1-based): thread (i+j-2) mod SB
 // Ownership of the coupling-matrix cells: the M rows are split into NBAND
-// bands; cell (i, j) of band b belongs to thread OW * b + ((i + j - 2) mod
-// OW).  Every row and every column then has distinct owners (M <= OW), so a
-// thread meets at most one cell of any line.
+// bands; cell (i, j) of band b belongs to thread
+//   OW * b + ((kOwnA * (i - 1) + (j - 1)) mod OW),
+// kOwnA odd.  Every row and every column then has distinct owners (M <= OW,
+// and kOwnA is invertible mod OW), so a thread meets at most one cell of any
+// line; the odd multiplier spreads a dense patch of cells (the low virtual
+// pairs most selectors land on) over the whole band instead of the few
+// threads of a diagonal stripe.
+constexpr int kOwnA = 37, kOwnAInv = 173;   // 37 * 173 = 1 mod 256
+static_assert((kOwnA * kOwnAInv) % OW == 1, "kOwnAInv is the inverse of kOwnA mod OW");
 static_assert(NBAND == 2, "band_of assumes two bands");
 __device__ inline int band_of(int i, int M) { return (2 * (i - 1) >= M) ? 1 : 0; }   // floor(2 (i-1) / M)
-__device__ inline bool owns(int i, int j, int M) {
-  return ((i + j - 2) & (OW - 1)) + OW * band_of(i, M) == (int)threadIdx.x;
+__device__ inline int owner_of(int i, int j, int M) {
+  return ((kOwnA * (i - 1) + (j - 1)) & (OW - 1)) + OW * band_of(i, M);
 }
+__device__ inline bool owns(int i, int j, int M) { return owner_of(i, j, M) == (int)threadIdx.x; }
 // the unique j (1..M) such that thread owns (i, j); 0 if none
 __device__ inline int owned_j(int i, int M) {
   if (band_of(i, M) != (int)threadIdx.x / OW) return 0;
-  const int j = (((int)threadIdx.x & (OW - 1)) - (i - 1)) & (OW - 1);
+  const int j = (((int)threadIdx.x & (OW - 1)) - kOwnA * (i - 1)) & (OW - 1);
   return (j < M) ? j + 1 : 0;
 }
 // the unique i (1..M) such that thread owns (i, j); 0 if none
 __device__ inline int owned_i(int j, int M) {
-  const int i0 = (((int)threadIdx.x & (OW - 1)) - (j - 1)) & (OW - 1);
+  const int i0 = (kOwnAInv * (((int)threadIdx.x & (OW - 1)) - (j - 1))) & (OW - 1);
   if (i0 >= M) return 0;
   return (band_of(i0 + 1, M) == (int)threadIdx.x / OW) ? i0 + 1 : 0;
 }
@@ -502,6 +567,8 @@ struct D2List {
 template <int NI>
 __device__ __forceinline__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const int (*h)[2], const int (*p)[4],
                                        const int *nh, const int *np, int sp, D2List &out) {
+  uint64_t pmk[2 * NI];
+  prefix_parity<NI>(sel, pmk);
   // p[s][k] = k+1-th particle of spin s+1; h similarly.  The (at most 6)
   // cells are collected first, their integrals loaded together, then the
   // values pushed in the reference's order: (V[ia] - V[ib]) * sign, or
@@ -540,7 +607,7 @@ __device__ __forceinline__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const in
         scj[k] = ma == 1 ? puti : putj;
         sia[k] = idx4(p1, p2, h1, h2);
         sib[k] = idx4(p2, p1, h1, h2);
-        sg[k] = phase_bi<NI>(sel, ma, ma, h1, p1, h2, p2);
+        sg[k] = phase_pm<NI>(pmk, ma, ma, h1, p1, h2, p2);
       }
     } else {
       const int h1 = H(1, 1), h2 = H(1, 2);
@@ -558,7 +625,7 @@ __device__ __forceinline__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const in
           scj[k] = putj;
           sia[k] = idx4(p1, p2, h1, h2);
           sib[k] = -1;
-          sg[k] = phase_bi<NI>(sel, 1, 2, h1, p1, h2, p2);
+          sg[k] = phase_pm<NI>(pmk, 1, 2, h1, p1, h2, p2);
         }
       }
     }
@@ -584,7 +651,7 @@ __device__ __forceinline__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const in
           scj[kk] = putj;
           sia[kk] = idx4(p1, p2, h1, h2);
           sib[kk] = idx4(p2, p1, h1, h2);
-          sg[kk] = phase_bi<NI>(sel, ma, ma, h1, p1, h2, p2);
+          sg[kk] = phase_pm<NI>(pmk, ma, ma, h1, p1, h2, p2);
         }
       }
     } else if (tip == 3) {
@@ -601,7 +668,7 @@ __device__ __forceinline__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const in
         scj[k] = max(puti, putj);
         sia[k] = idx4(p1, p2, h1, h2);
         sib[k] = -1;
-        sg[k] = phase_bi<NI>(sel, mi, ma, h1, p1, h2, p2);
+        sg[k] = phase_pm<NI>(pmk, mi, ma, h1, p1, h2, p2);
       }
     } else {
       const int puti = P(1, sp), putj = P(2, sp);
@@ -612,7 +679,7 @@ __device__ __forceinline__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const in
         scj[0] = putj;
         sia[0] = idx4(p1, p2, h1, h2);
         sib[0] = idx4(p2, p1, h1, h2);
-        sg[0] = phase_bi<NI>(sel, mi, mi, h1, p1, h2, p2);
+        sg[0] = phase_pm<NI>(pmk, mi, mi, h1, p1, h2, p2);
       }
     }
   }
@@ -642,6 +709,8 @@ template <int NI>
 __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const int (*h)[2], const int (*p)[4],
                                        const int *nh, const int *np, int sp, const double *coefs, int na_e,
                                        int nb_e) {
+  uint64_t pmk[2 * NI];
+  prefix_parity<NI>(sel, pmk);
   const int M = C.M;
   const uint32_t *BO = C.L.borb;
   const uint32_t *B1 = C.L.ban1;
@@ -664,9 +733,10 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const in
   double sg[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) sv[k] = false;
-  auto idx4 = [&](int i, int j, int k, int l) -> int64_t {   // <ij|kl>, 1-based
-    return pair_index(i - 1, k - 1) * (int64_t)C.T.npair + pair_index(j - 1, l - 1);
-  };
+  // <ij|kl> (1-based) with argument `vary` the orbital that changes along
+  // the line (the cells' owners are consecutive threads)
+  auto idx4 = [&](int i, int j, int k, int l, int vary) -> int64_t { return line_off(C.T, i, j, k, l, vary); };
+  const double *TT = C.T.G4 ? C.T.G4 : C.T.V;
   auto row_of = [&](int j) -> int { return owned_i(j, M); };   // i with owns(i, j), 0 if none
   if (sp == 3) {
     const uint32_t *BB = (ma == 2) ? C.L.ban2 : C.L.ban1;
@@ -678,13 +748,13 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const in
         sci[0] = (ma == 1) ? putj : puti;
         scj[0] = (ma == 1) ? puti : putj;
         if (putj < hfix) {
-          sia[0] = idx4(p1, p2, putj, hfix);
-          sib[0] = idx4(p2, p1, putj, hfix);
-          sg[0] = phase_bi<NI>(sel, ma, ma, putj, p1, hfix, p2);
+          sia[0] = idx4(p1, p2, putj, hfix, 2);
+          sib[0] = idx4(p2, p1, putj, hfix, 2);
+          sg[0] = phase_pm<NI>(pmk, ma, ma, putj, p1, hfix, p2);
         } else {
-          sia[0] = idx4(p1, p2, hfix, putj);
-          sib[0] = idx4(p2, p1, hfix, putj);
-          sg[0] = phase_bi<NI>(sel, ma, ma, hfix, p1, putj, p2);
+          sia[0] = idx4(p1, p2, hfix, putj, 3);
+          sib[0] = idx4(p2, p1, hfix, putj, 3);
+          sg[0] = phase_pm<NI>(pmk, ma, ma, hfix, p1, putj, p2);
         }
       }
     }
@@ -700,9 +770,9 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const in
       sv[1 + which] = true;
       sci[1 + which] = (mi == 1) ? pu : pf;
       scj[1 + which] = (mi == 1) ? pf : pu;
-      sia[1 + which] = idx4(other, pfix, hfix, pu);
+      sia[1 + which] = idx4(other, pfix, hfix, pu, 3);
       sib[1 + which] = -1;
-      sg[1 + which] = phase_bi<NI>(sel, ma, mi, hfix, other, pu, pfix);
+      sg[1 + which] = phase_pm<NI>(pmk, ma, mi, hfix, other, pu, pfix);
     }
   } else if (np[ma - 1] == 3) {
     const int hfix = H(1, ma);
@@ -730,13 +800,13 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const in
         sci[k] = half == 0 ? x : puti;
         scj[k] = half == 0 ? puti : x;
         if (putj < hfix) {
-          sia[k] = idx4(p1, p2, putj, hfix);
-          sib[k] = idx4(p2, p1, putj, hfix);
-          sg[k] = phase_bi<NI>(sel, ma, ma, putj, p1, hfix, p2);
+          sia[k] = idx4(p1, p2, putj, hfix, 2);
+          sib[k] = idx4(p2, p1, putj, hfix, 2);
+          sg[k] = phase_pm<NI>(pmk, ma, ma, putj, p1, hfix, p2);
         } else {
-          sia[k] = idx4(p1, p2, hfix, putj);
-          sib[k] = idx4(p2, p1, hfix, putj);
-          sg[k] = phase_bi<NI>(sel, ma, ma, hfix, p1, putj, p2);
+          sia[k] = idx4(p1, p2, hfix, putj, 3);
+          sib[k] = idx4(p2, p1, hfix, putj, 3);
+          sg[k] = phase_pm<NI>(pmk, ma, ma, hfix, p1, putj, p2);
         }
       }
     }
@@ -763,9 +833,9 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const in
         sv[k] = true;
         sci[k] = half == 0 ? x : pc;
         scj[k] = half == 0 ? pc : x;
-        sia[k] = idx4(pfix, other, hfix, puti);
+        sia[k] = idx4(pfix, other, hfix, puti, 3);
         sib[k] = -1;
-        sg[k] = phase_bi<NI>(sel, mi, ma, hfix, pfix, puti, other);
+        sg[k] = phase_pm<NI>(pmk, mi, ma, hfix, pfix, puti, other);
       }
     }
   }
@@ -773,8 +843,8 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const in
     double va[6], vb[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-      va[k] = sv[k] ? C.T.V[sia[k]] : 0.0;
-      vb[k] = (sv[k] && sib[k] >= 0) ? C.T.V[sib[k]] : 0.0;
+      va[k] = sv[k] ? TT[sia[k]] : 0.0;
+      vb[k] = (sv[k] && sib[k] >= 0) ? TT[sib[k]] : 0.0;
     }
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
@@ -833,6 +903,9 @@ __device__ __forceinline__ void get_d0(Ctx<NI> &C, const uint64_t *sel, const in
     C.add(i, j, coefs, i_H_j<NI>(C.T, sel, det, na_e, nb_e));
   };
   auto col_owner_row = [&](int j) -> int { return owned_i(j, M); };   // i with owns(i, j), 0 if none
+  // the cells of a row belong to consecutive threads: their integrals are
+  // read along the column orbital
+  const double *TT = C.T.G4 ? C.T.G4 : C.T.V;
   int r_lo, r_hi;
   band_rows(M, r_lo, r_hi);
   if (sp == 3) {
@@ -849,7 +922,7 @@ __device__ __forceinline__ void get_d0(Ctx<NI> &C, const uint64_t *sel, const in
                        !(borb_get(BO, p1, 1) || borb_get(BO, p2, 2) || ban_get(B1, M, p1, p2));
         cj[b] = p2;
         ok[b] = o;
-        v[b] = o ? I4(C.T, p1, p2, h1, h2) : 0.0;
+        v[b] = o ? TT[line_off(C.T, p1, p2, h1, h2, 1)] : 0.0;
       }
 #pragma unroll
       for (int b = 0; b < RB8; ++b) {
@@ -881,8 +954,8 @@ __device__ __forceinline__ void get_d0(Ctx<NI> &C, const uint64_t *sel, const in
                        !(borb_get(BO, puti, sp) || borb_get(BO, putj, sp) || ban_get(B1, M, puti, putj));
         cj[b] = putj;
         ok[b] = o;
-        va[b] = o ? I4(C.T, p1, p2, puti, putj) : 0.0;
-        vb[b] = o ? I4(C.T, p2, p1, puti, putj) : 0.0;
+        va[b] = o ? TT[line_off(C.T, p1, p2, puti, putj, 3)] : 0.0;
+        vb[b] = o ? TT[line_off(C.T, p2, p1, puti, putj, 3)] : 0.0;
       }
 #pragma unroll
       for (int b = 0; b < RB8; ++b) {
@@ -1011,8 +1084,11 @@ struct QueueLds {
   }
 };
 
-// owner thread of cell (i, j): OW * band(i) + (i + j - 2) mod OW
-__device__ inline int owner_of(int i, int j, int M) { return ((i + j - 2) & (OW - 1)) + OW * band_of(i, M); }
+// a queue entry of three or fewer particles keeps its determinant in the
+// six coupling slots (48 bytes) when it fits there (N_int <= 3)
+template <int NI>
+constexpr bool kQueueHoldsDet = 2 * NI * sizeof(uint64_t) <= 6 * sizeof(double);
+
 
 // GM = false: the coupling matrix lives in LDS.  GM = true (N_states * M^2
 // doubles do not fit next to the queue): it lives in HBM, one slot of
@@ -1336,7 +1412,8 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
       apply_until(k);
       const int kind = Q.kind_n[k] & 0xf;
       uint64_t d[2 * NI];
-      const uint64_t *dp = A.psi + (int64_t)(Q.idx[k] - 1) * 2 * NI;
+      const uint64_t *dp = kQueueHoldsDet<NI> ? reinterpret_cast<const uint64_t *>(Q.hij + 6 * k)
+                                              : A.psi + (int64_t)(Q.idx[k] - 1) * 2 * NI;
 #pragma unroll
       for (int w = 0; w < 2 * NI; ++w) d[w] = dp[w];
       int p[2][4], np[2], h[2][2], nh[2];
@@ -1404,6 +1481,12 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
           int p[2][4], np[2], h[2][2], nh[2];
           sel_lists<NI>(mask, d, p, np, h, nh);
           get_d2<NI>(C, d, h, p, nh, np, sp, dl);
+        } else if (kQueueHoldsDet<NI>) {
+          // three or fewer particles: the determinant goes where the
+          // four-particle couplings would be, for the replay to read from LDS
+          uint64_t *qd = reinterpret_cast<uint64_t *>(Q.hij + 6 * e);
+#pragma unroll
+          for (int w = 0; w < 2 * NI; ++w) qd[w] = d[w];
         }
         Q.kind_n[e] = (nt == 4 ? 0 : (nt == 3 ? 1 : 2)) | (dl.n << 4);
       }
@@ -1589,7 +1672,9 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
 
   DBuf<double> coef_d;
   coef_d.upload(coef, (size_t)n_det * NS, st);
-  Tabs T{H.V.p, H.hcore.p, H.Jd.p, H.JAd.p, H.fock_cs.p, M, H.npair, H.ref_energy};
+  // the M^4 table (shared with the sparse sigma path), built on first use
+  const bool g4 = ensure_g4(c);
+  Tabs T{H.V.p, H.hcore.p, H.Jd.p, H.JAd.p, H.fock_cs.p, M, H.npair, H.ref_energy, g4 ? H.G4.p : nullptr};
 
   // hole lists = occupied orbitals of each generator (full generator masks)
   std::vector<uint64_t> psi_h((size_t)n_det * 2 * NI);
