@@ -80,7 +80,7 @@ def main():
                     help="selected-psi leg: sigma timings only (for counter runs)")
     ap.add_argument("--no-mlp", action="store_true")
     ap.add_argument("--no-cipsi", action="store_true")
-    ap.add_argument("--cipsi-ndet", type=int, default=32768)
+    ap.add_argument("--cipsi-ndet", type=int, default=65536)
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()

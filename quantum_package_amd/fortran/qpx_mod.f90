@@ -75,6 +75,17 @@ module qpx_mod
       real(c_double) :: coef(*), e0(*), genpt2(*)
       integer(c_int) :: gens(*)
     end function
+    integer(c_int) function qpx_select_tasks(ctx, nstates, coef, nsel, e0, nkeep, ntasks, tgen, tsub, pt2, dets, &
+        vals, nout, taskpt2) bind(C, name="qpx_select_tasks")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nstates, nsel, ntasks
+      integer(c_int64_t), value :: nkeep
+      real(c_double) :: coef(*), e0(*), pt2(*), vals(*), taskpt2(*)
+      integer(c_int) :: tgen(*), tsub(*)
+      integer(c_int64_t) :: dets(*)
+      integer(c_int64_t) :: nout
+    end function
     integer(c_int) function qpx_make_s2_eigenfunction(nint, ndet, psi, nalpha, maxout, dets, nout) &
         bind(C, name="qpx_make_s2_eigenfunction")
       import :: c_int, c_int64_t
@@ -177,6 +188,25 @@ contains
     g = int(gens, c_int)
     call qpx_check(qpx_select_generators(qpx_ctx, int(N_states, c_int), psi_coef_sorted, &
         int(N_det_selectors, c_int), E0, int(n_list, c_int), g, pt2_detail))
+  end subroutine
+
+  ! The selection task farm (zmq_selection.irp.f:54-100) over an explicit
+  ! list of "i_generator subset" tasks (run_selection_slave.irp.f:46-106) on
+  ! psi_det_sorted, with one merged buffer; task_pt2(:, t) is each task's pt2
+  subroutine qpx_selection_tasks(N_in, pt2, psi_coef_sorted, ndet, N_states, N_det_selectors, E0, nint, &
+      n_tasks, task_generator, task_subset, sel_dets, sel_vals, n_sel_out, task_pt2)
+    integer, intent(in) :: N_in, ndet, N_states, N_det_selectors, nint, n_tasks
+    integer, intent(in) :: task_generator(n_tasks), task_subset(n_tasks)
+    double precision, intent(in) :: psi_coef_sorted(ndet, N_states), E0(N_states)
+    double precision, intent(out) :: pt2(N_states), sel_vals(N_in), task_pt2(N_states, n_tasks)
+    integer(8), intent(out) :: sel_dets(nint, 2, N_in)
+    integer(8), intent(out) :: n_sel_out
+    integer(c_int), allocatable :: g(:), sb(:)
+    allocate(g(n_tasks), sb(n_tasks))
+    g = int(task_generator, c_int)
+    sb = int(task_subset, c_int)
+    call qpx_check(qpx_select_tasks(qpx_ctx, int(N_states, c_int), psi_coef_sorted, int(N_det_selectors, c_int), &
+        E0, int(N_in, c_int64_t), int(n_tasks, c_int), g, sb, pt2, sel_dets, sel_vals, n_sel_out, task_pt2))
   end subroutine
 
   ! make_s2_eigenfunction (occ_pattern.irp.f:251-309): the determinants that
