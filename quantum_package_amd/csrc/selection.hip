@@ -380,18 +380,22 @@ struct NearArgs {
   const int64_t *off;  // fill offsets (per generator)
   int *cnt;            // counts (per generator)
   int *list;           // output det indices (1-based)
+  int *past;           // per generator: kept entries with index < the generator
 };
 
 template <int NI>
 __global__ __launch_bounds__(256) void near_kernel(NearArgs A, int fill) {
-  __shared__ int wcount[4];
-  __shared__ int base;
+  __shared__ int wcount[4], wbefore[4];
+  __shared__ int base, before_gen;
   const int g = A.gens[blockIdx.x];
   const uint64_t *gen = A.psi + (int64_t)(g - 1) * 2 * NI;
   uint64_t neg[2 * NI];
 #pragma unroll
   for (int w = 0; w < 2 * NI; ++w) neg[w] = ~gen[w];
-  if (threadIdx.x == 0) base = 0;
+  if (threadIdx.x == 0) {
+    base = 0;
+    before_gen = 0;
+  }
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int64_t out0 = fill ? A.off[blockIdx.x] : 0;
@@ -406,8 +410,12 @@ __global__ __launch_bounds__(256) void near_kernel(NearArgs A, int fill) {
       keep = (i + 1 <= A.n_sel) ? (nt <= 4) : (nt <= 2);
     }
     uint64_t m = __ballot(keep);
+    const uint64_t mb = __ballot(keep && i + 1 < g);   // kept entries before the generator
     int my = __popcll(m & ((1ULL << lane) - 1));
-    if (lane == 0) wcount[wv] = __popcll(m);
+    if (lane == 0) {
+      wcount[wv] = __popcll(m);
+      wbefore[wv] = __popcll(mb);
+    }
     __syncthreads();
     int pre = 0, tot = 0;
     for (int x = 0; x < 4; ++x) {
@@ -416,10 +424,15 @@ __global__ __launch_bounds__(256) void near_kernel(NearArgs A, int fill) {
     }
     if (fill && keep) A.list[out0 + base + pre + my] = (int)(i + 1);
     __syncthreads();
-    if (threadIdx.x == 0) base += tot;
+    if (threadIdx.x == 0) {
+      base += tot;
+      for (int x = 0; x < 4; ++x) before_gen += wbefore[x];
+    }
     __syncthreads();
   }
   if (!fill && threadIdx.x == 0) A.cnt[blockIdx.x] = base;
+  // the kept entries before the generator form a prefix of the list
+  if (!fill && threadIdx.x == 0) A.past[blockIdx.x] = before_gen;
 }
 
 // ---------------------------------------------------------------- blocks
@@ -458,7 +471,89 @@ struct SelArgs {
   int64_t *blk_ngath;
   int64_t *blk_ticks;  // duration of the workgroup in 100 MHz wall-clock ticks (cost model of the proposal)
   double *mat_g;       // coupling matrices in HBM (one NS*M*M slot per workgroup), when they do not fit LDS
+  const double *fockg; // fock_diag_tmp of each generator of the batch, 2*(M+1) per near-list slot
+  const int *near_past;   // per near-list slot: entries with index < the generator (a prefix of the list)
 };
+
+// fock_diag_tmp (Fock_diag.irp.f:1-93) of each generator of a batch: the
+// diagonal Fock energies of both spins (slot M+1 holds E_gen), as
+// F_[(s-1) + 2*(i-1)], one workgroup per generator.  Every block of the
+// generator copies it.
+template <int NI>
+__global__ __launch_bounds__(256) void fock_gen_kernel(const uint64_t *__restrict__ psi, const int *__restrict__ gens,
+                                                      Tabs T, double *__restrict__ fockg) {
+  const int M = T.M, tid = threadIdx.x;
+  const uint64_t *gen = psi + (int64_t)(gens[blockIdx.x] - 1) * 2 * NI;
+  double *F_ = fockg + (size_t)blockIdx.x * 2 * (M + 1);
+    const uint64_t *ga = gen, *gb = gen + NI;
+#define F(s, i) F_[((s) - 1) + 2 * ((i) - 1)]
+#define JA(a, b) T.JA[((a) - 1) + M * ((b) - 1)]
+#define JJ(a, b) T.J[((a) - 1) + M * ((b) - 1)]
+    for (int x = tid; x < 2 * M; x += blockDim.x) {
+      int s = x / M + 1, i = x % M + 1;
+      const uint64_t *same = s == 1 ? ga : gb, *oth = s == 1 ? gb : ga;
+      double f = 0.0;
+      if (bit(same, i)) {
+        f += T.hcore[(i - 1) * (M + 1)];
+        for (int w = 0; w < NI; ++w) {
+          uint64_t y = same[w];
+          while (y) { int j = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; if (j != i) f += JA(i, j); }
+        }
+        for (int w = 0; w < NI; ++w) {
+          uint64_t y = oth[w];
+          while (y) { int j = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; f += JJ(i, j); }
+        }
+      }
+      if (f == 0.0) {
+        f += T.hcore[(i - 1) * (M + 1)];
+        for (int w = 0; w < NI; ++w) {
+          uint64_t y = same[w];
+          while (y) { int j = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; f += JA(i, j); }
+        }
+        for (int w = 0; w < NI; ++w) {
+          uint64_t y = oth[w];
+          while (y) { int j = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; f += JJ(i, j); }
+        }
+      }
+      F(s, i) = f;
+    }
+    if (tid == 0) {
+      double E0 = 0.0;
+      for (int w = 0; w < NI; ++w) {
+        uint64_t y = ga[w];
+        while (y) {
+          int i = 64 * w + __ffsll((unsigned long long)y);
+          y &= y - 1;
+          E0 += T.hcore[(i - 1) * (M + 1)];
+          for (int w2 = 0; w2 < NI; ++w2) {
+            uint64_t z = ga[w2];
+            while (z) { int j = 64 * w2 + __ffsll((unsigned long long)z); z &= z - 1; if (j != i) E0 += 0.5 * JA(i, j); }
+          }
+          for (int w2 = 0; w2 < NI; ++w2) {
+            uint64_t z = gb[w2];
+            while (z) { int j = 64 * w2 + __ffsll((unsigned long long)z); z &= z - 1; E0 += JJ(i, j); }
+          }
+        }
+      }
+      for (int w = 0; w < NI; ++w) {
+        uint64_t y = gb[w];
+        while (y) {
+          int i = 64 * w + __ffsll((unsigned long long)y);
+          y &= y - 1;
+          E0 += T.hcore[(i - 1) * (M + 1)];
+          for (int w2 = 0; w2 < NI; ++w2) {
+            uint64_t z = gb[w2];
+            while (z) { int j = 64 * w2 + __ffsll((unsigned long long)z); z &= z - 1; if (j != i) E0 += 0.5 * JA(i, j); }
+          }
+        }
+      }
+      F(1, M + 1) = E0;
+      F(2, M + 1) = E0;
+    }
+#undef F
+#undef JA
+#undef JJ
+  }
 
 template <int NI>
 __device__ inline void build_mask(const uint64_t *gen, int s1, int h1, int s2, int h2, uint64_t *mask) {
@@ -1140,77 +1235,8 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
     L.borb[tid] = v;
   }
   if (tid == 0) L.flag[0] = 0;
-  // fock_diag_tmp (Fock_diag.irp.f:1-93)
-  {
-    const uint64_t *ga = gen, *gb = gen + NI;
-#define F(s, i) L.fock[((s) - 1) + 2 * ((i) - 1)]
-#define JA(a, b) T.JA[((a) - 1) + M * ((b) - 1)]
-#define JJ(a, b) T.J[((a) - 1) + M * ((b) - 1)]
-    for (int x = tid; x < 2 * M; x += SB) {
-      int s = x / M + 1, i = x % M + 1;
-      const uint64_t *same = s == 1 ? ga : gb, *oth = s == 1 ? gb : ga;
-      double f = 0.0;
-      if (bit(same, i)) {
-        f += T.hcore[(i - 1) * (M + 1)];
-        for (int w = 0; w < NI; ++w) {
-          uint64_t y = same[w];
-          while (y) { int j = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; if (j != i) f += JA(i, j); }
-        }
-        for (int w = 0; w < NI; ++w) {
-          uint64_t y = oth[w];
-          while (y) { int j = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; f += JJ(i, j); }
-        }
-      }
-      if (f == 0.0) {
-        f += T.hcore[(i - 1) * (M + 1)];
-        for (int w = 0; w < NI; ++w) {
-          uint64_t y = same[w];
-          while (y) { int j = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; f += JA(i, j); }
-        }
-        for (int w = 0; w < NI; ++w) {
-          uint64_t y = oth[w];
-          while (y) { int j = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; f += JJ(i, j); }
-        }
-      }
-      F(s, i) = f;
-    }
-    if (tid == 0) {
-      double E0 = 0.0;
-      for (int w = 0; w < NI; ++w) {
-        uint64_t y = ga[w];
-        while (y) {
-          int i = 64 * w + __ffsll((unsigned long long)y);
-          y &= y - 1;
-          E0 += T.hcore[(i - 1) * (M + 1)];
-          for (int w2 = 0; w2 < NI; ++w2) {
-            uint64_t z = ga[w2];
-            while (z) { int j = 64 * w2 + __ffsll((unsigned long long)z); z &= z - 1; if (j != i) E0 += 0.5 * JA(i, j); }
-          }
-          for (int w2 = 0; w2 < NI; ++w2) {
-            uint64_t z = gb[w2];
-            while (z) { int j = 64 * w2 + __ffsll((unsigned long long)z); z &= z - 1; E0 += JJ(i, j); }
-          }
-        }
-      }
-      for (int w = 0; w < NI; ++w) {
-        uint64_t y = gb[w];
-        while (y) {
-          int i = 64 * w + __ffsll((unsigned long long)y);
-          y &= y - 1;
-          E0 += T.hcore[(i - 1) * (M + 1)];
-          for (int w2 = 0; w2 < NI; ++w2) {
-            uint64_t z = gb[w2];
-            while (z) { int j = 64 * w2 + __ffsll((unsigned long long)z); z &= z - 1; if (j != i) E0 += 0.5 * JA(i, j); }
-          }
-        }
-      }
-      F(1, M + 1) = E0;
-      F(2, M + 1) = E0;
-    }
-#undef F
-#undef JA
-#undef JJ
-  }
+  // fock_diag_tmp of the generator (computed once per generator by fock_gen_kernel)
+  for (int x = tid; x < 2 * (M + 1); x += SB) L.fock[x] = A.fockg[(size_t)B.gslot * 2 * (M + 1) + x];
   __syncthreads();
   if (tid == 0) {
     if (B.unbanA) L.borb[(B.s1 - 1) * 8 + ((B.h1 - 1) >> 5)] &= ~(1u << ((B.h1 - 1) & 31));
@@ -1256,7 +1282,9 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
     return;
   }
   // ---- past_d1 / past_d2 (selectors before the generator) ----
-  for (int x = tid; x < n_near; x += SB) {
+  // the near list is in index order: those selectors are its first entries
+  const int n_past = A.near_past[B.gslot];
+  for (int x = tid; x < n_past; x += SB) {
     const int idx = near[x];
     if (idx >= B.gen || idx > A.n_sel) continue;
     const uint64_t *d = A.psi + (int64_t)(idx - 1) * 2 * NI;
@@ -1437,9 +1465,11 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
       for (int w = 0; w < 2 * NI; ++w) nx_d[w] = d[w];
     }
   };
-  fetch(tid);
+  // the selectors before the generator (the first n_past entries of the
+  // near list) never pass the filter: the scan starts after them
+  fetch(n_past + tid);
   // one extra round past the end of the list flushes what is left
-  for (int x0 = 0;; x0 += SB) {
+  for (int x0 = n_past;; x0 += SB) {
     const bool more = x0 < n_near;
     const int idx = nx_idx;
     uint64_t d[2 * NI];
@@ -1765,7 +1795,8 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
   std::vector<int64_t> blk_ticks_all(n_blocks_total, 0);
   int64_t ncand_total = 0, ngath_total = 0, nproc = 0;
 
-  DBuf<int> gens_d, near_cnt_d, near_list_d;
+  DBuf<int> gens_d, near_cnt_d, near_list_d, near_past_d;
+  DBuf<double> fockg_d;
   DBuf<int64_t> near_off_d;
   DBuf<BlockDesc> blocks_d;
   DBuf<double> bpt2_d;
@@ -1835,7 +1866,8 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
     gens_d.upload(gens, st);
     near_cnt_d.alloc(ng);
     near_off_d.alloc(ng);
-    NearArgs NA{W.psi.p, n_det, n_sel, gens_d.p, ng, near_off_d.p, near_cnt_d.p, nullptr};
+    near_past_d.alloc(ng);
+    NearArgs NA{W.psi.p, n_det, n_sel, gens_d.p, ng, near_off_d.p, near_cnt_d.p, nullptr, near_past_d.p};
     hipLaunchKernelGGL(near_kernel<NI>, dim3(ng), dim3(256), 0, st, NA, 0);
     QPX_HIP(hipGetLastError());
     std::vector<int> cnt(ng);
@@ -1848,6 +1880,9 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
     near_list_d.alloc(std::max<int64_t>(tot, 1));
     NA.list = near_list_d.p;
     hipLaunchKernelGGL(near_kernel<NI>, dim3(ng), dim3(256), 0, st, NA, 1);
+    QPX_HIP(hipGetLastError());
+    fockg_d.alloc((size_t)ng * 2 * (M + 1));
+    hipLaunchKernelGGL(fock_gen_kernel<NI>, dim3(ng), dim3(256), 0, st, W.psi.p, gens_d.p, T, fockg_d.p);
     QPX_HIP(hipGetLastError());
     // blocks of this batch
     std::vector<BlockDesc> bl;
@@ -1881,6 +1916,7 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
       for (int s = 0; s < NS; ++s) SA.E0[s] = E0[s];
       SA.blocks = blocks_d.p; SA.n_blocks = nb;
       SA.near_off = near_off_d.p; SA.near_cnt = near_cnt_d.p; SA.near_list = near_list_d.p;
+      SA.near_past = near_past_d.p; SA.fockg = fockg_d.p;
       SA.pool_key = P.key.p; SA.pool_det = P.det.p; SA.pool_blk = P.blk.p; SA.pool_count = P.count.p;
       SA.pool_cap = P.cap; SA.tau = tau;
       SA.blk_pt2 = bpt2_d.p; SA.blk_status = bstat_d.p; SA.blk_ncand = bncand_d.p; SA.blk_ngath = bngath_d.p;
