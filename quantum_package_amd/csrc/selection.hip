@@ -13,15 +13,19 @@
 //     preinteresting / prefullinteresting lists);
 //   * one workgroup owns one (generator, h1, h2) block -- the reference's
 //     maskInd unit.  The block's coupling matrix mat(N_states, M, M) lives
-//     in LDS; with the rows split into two bands, cell (i,j) is owned by
-//     thread 256 * band(i) + (i+j-2) mod 256, so every line of cells
-//     touched by get_d1 maps to distinct threads and each cell receives its
-//     terms in selector order, the same order as the reference
-//     (bit-identical FP64 sums);
+//     in LDS (in an HBM slot per workgroup when it leaves no room for a
+//     queue); with the rows split into two bands, cell (i,j) is owned by
+//     thread 256 * band(i) + (37 (i-1) + (j-1)) mod 256, so every row and
+//     every column of cells maps to distinct threads;
 //   * the block's selectors are filtered in parallel into an LDS queue in
-//     near-list order; the queue is then replayed in order, the couplings
-//     of four-particle selectors having been computed by the filtering
-//     thread;
+//     near-list order.  The filtering thread computes the <= 6 couplings of
+//     a four-particle selector (get_d2) into its queue entry and stores the
+//     determinant of any other survivor.  The replay turns the four-particle
+//     couplings into per-owner record lists (count, scan, scatter, sort);
+//     each thread then walks the get_d1 / get_d0 entries in queue order and
+//     applies its earlier records before each one, so every cell receives
+//     its terms in selector order, the same order as the reference
+//     (bit-identical FP64 sums);
 //   * bans (spot_isinwf, past_d1/d2, symmetrisation) are order-independent
 //     sets and are built in parallel with LDS atomics before any coupling is
 //     accumulated;
