@@ -65,23 +65,59 @@ __device__ inline bool bit(const uint64_t *s, int o) {  // o 1-based
   return (s[(o - 1) >> 6] >> ((o - 1) & 63)) & 1;
 }
 
-// parity of occupied orbitals with index <= o in string s (get_mask_phase)
+// Determinant-shaped arrays (2 * NI words, spin 1 then spin 2) that live in
+// registers are only ever indexed with compile-time word numbers: a single
+// run-time index into such an array places all of it in scratch memory.
+// These helpers pick the word with selects instead.
 template <int NI>
-__device__ inline int pm(const uint64_t *s, int o) {
+__device__ __forceinline__ uint64_t spin_word(const uint64_t *a, int s, int w) {   // w a constant
+  return s == 1 ? a[w] : a[NI + w];
+}
+// the word of spin s (1-based) holding orbital o (1-based)
+template <int NI>
+__device__ __forceinline__ uint64_t orb_word(const uint64_t *a, int s, int o) {
+  const int wo = (o - 1) >> 6;
+  uint64_t x = spin_word<NI>(a, s, 0);
+#pragma unroll
+  for (int w = 1; w < NI; ++w)
+    if (wo == w) x = spin_word<NI>(a, s, w);
+  return x;
+}
+template <int NI>
+__device__ __forceinline__ void set_orb(uint64_t *a, int s, int o) {
+  const int idx = (s - 1) * NI + ((o - 1) >> 6);
+  const uint64_t b = 1ULL << ((o - 1) & 63);
+#pragma unroll
+  for (int w = 0; w < 2 * NI; ++w)
+    if (w == idx) a[w] |= b;
+}
+template <int NI>
+__device__ __forceinline__ void clear_orb(uint64_t *a, int s, int o) {
+  const int idx = (s - 1) * NI + ((o - 1) >> 6);
+  const uint64_t b = 1ULL << ((o - 1) & 63);
+#pragma unroll
+  for (int w = 0; w < 2 * NI; ++w)
+    if (w == idx) a[w] &= ~b;
+}
+
+// parity of the occupied orbitals with index <= o in spin s of det
+// (get_mask_phase)
+template <int NI>
+__device__ inline int pm(const uint64_t *det, int s, int o) {
   int c = 0;
 #pragma unroll
   for (int w = 0; w < NI; ++w) {
-    int lo = 64 * w;
-    if (o - 1 >= lo + 63) c += __popcll(s[w]);
-    else if (o - 1 >= lo) c += __popcll(s[w] & ((2ULL << ((o - 1) - lo)) - 1));
+    const int lo = 64 * w;
+    const uint64_t x = spin_word<NI>(det, s, w);
+    if (o - 1 >= lo + 63) c += __popcll(x);
+    else if (o - 1 >= lo) c += __popcll(x & ((2ULL << ((o - 1) - lo)) - 1));
   }
   return c & 1;
 }
 
 template <int NI>
 __device__ inline double phase_bi(const uint64_t *sel, int s1, int s2, int h1, int p1, int h2, int p2) {
-  const uint64_t *a = sel + (s1 - 1) * NI, *b = sel + (s2 - 1) * NI;
-  int np = pm<NI>(a, h1) + pm<NI>(a, p1) + pm<NI>(b, h2) + pm<NI>(b, p2);
+  int np = pm<NI>(sel, s1, h1) + pm<NI>(sel, s1, p1) + pm<NI>(sel, s2, h2) + pm<NI>(sel, s2, p2);
   if (p1 < h1) np++;
   if (p2 < h2) np++;
   if (s1 == s2 && max(h1, p1) > min(h2, p2)) np++;
@@ -112,13 +148,13 @@ __device__ inline void prefix_parity(const uint64_t *sel, uint64_t *pmk) {
   }
 }
 
-__device__ inline int pm_bit(const uint64_t *pmk_s, int o) { return (int)((pmk_s[(o - 1) >> 6] >> ((o - 1) & 63)) & 1); }
+template <int NI>
+__device__ inline int pm_bit(const uint64_t *pmk, int s, int o) { return (int)((orb_word<NI>(pmk, s, o) >> ((o - 1) & 63)) & 1); }
 
 // phase_bi from the prefix-parity masks
 template <int NI>
 __device__ inline double phase_pm(const uint64_t *pmk, int s1, int s2, int h1, int p1, int h2, int p2) {
-  const uint64_t *a = pmk + (s1 - 1) * NI, *b = pmk + (s2 - 1) * NI;
-  int np = pm_bit(a, h1) + pm_bit(a, p1) + pm_bit(b, h2) + pm_bit(b, p2);
+  int np = pm_bit<NI>(pmk, s1, h1) + pm_bit<NI>(pmk, s1, p1) + pm_bit<NI>(pmk, s2, h2) + pm_bit<NI>(pmk, s2, p2);
   if (p1 < h1) np++;
   if (p2 < h2) np++;
   if (s1 == s2 && max(h1, p1) > min(h2, p2)) np++;
@@ -157,54 +193,61 @@ __device__ inline double I4(const Tabs &T, int i, int j, int k, int l) {  // <ij
   return T.V[pair_index(i - 1, k - 1) * T.npair + pair_index(j - 1, l - 1)];
 }
 
+// holes/particles of spin s between determinants a (initial) and b (final),
+// 1-based, ascending (the first two of each)
+struct SpinExc {
+  int nh, np, h0, h1, p0, p1;
+};
 template <int NI>
-__device__ inline int count_between1(const uint64_t *a, int h, int p) {  // 1-based, strictly between
+__device__ inline SpinExc spin_exc(const uint64_t *a, const uint64_t *b, int s) {
+  SpinExc e{0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int w = 0; w < NI; ++w) {
+    const uint64_t aw = spin_word<NI>(a, s, w), bw = spin_word<NI>(b, s, w);
+    const uint64_t x = aw ^ bw;
+    uint64_t pp = x & bw, hh = x & aw;
+    while (pp) {
+      const int o = 64 * w + __ffsll((unsigned long long)pp);
+      pp &= pp - 1;
+      if (e.np == 0) e.p0 = o;
+      else if (e.np == 1) e.p1 = o;
+      e.np++;
+    }
+    while (hh) {
+      const int o = 64 * w + __ffsll((unsigned long long)hh);
+      hh &= hh - 1;
+      if (e.nh == 0) e.h0 = o;
+      else if (e.nh == 1) e.h1 = o;
+      e.nh++;
+    }
+  }
+  return e;
+}
+
+// occupied orbitals of spin s of det strictly between h and p (1-based)
+template <int NI>
+__device__ inline int count_between1(const uint64_t *det, int s, int h, int p) {
   int lo = min(h, p), hi = max(h, p);
   int c = 0;
 #pragma unroll
   for (int w = 0; w < NI; ++w) {
-    int s = lo - 64 * w, e = hi - 1 - 64 * w;  // 0-based bit positions lo..hi-2 <=> orbitals lo+1..hi-1
-    if (s < 0) s = 0;
+    int b0 = lo - 64 * w, e = hi - 1 - 64 * w;  // 0-based bit positions lo..hi-2 <=> orbitals lo+1..hi-1
+    if (b0 < 0) b0 = 0;
     if (e > 64) e = 64;
-    if (s < e) {
-      uint64_t m = ((e == 64) ? ~0ULL : ((1ULL << e) - 1)) & ~((1ULL << s) - 1);
-      c += __popcll(a[w] & m);
+    if (b0 < e) {
+      uint64_t m = ((e == 64) ? ~0ULL : ((1ULL << e) - 1)) & ~((1ULL << b0) - 1);
+      c += __popcll(spin_word<NI>(det, s, w) & m);
     }
   }
   return c;
 }
 
-// holes/particles of one spin between strings a (initial) and b (final),
-// 1-based, ascending
 template <int NI>
-__device__ inline void spin_lists(const uint64_t *a, const uint64_t *b, int *nh, int *hl, int *np, int *pl) {
-  *nh = 0;
-  *np = 0;
-#pragma unroll
-  for (int w = 0; w < NI; ++w) {
-    uint64_t x = a[w] ^ b[w];
-    uint64_t pp = x & b[w], hh = x & a[w];
-    while (pp) {
-      int o = 64 * w + __ffsll((unsigned long long)pp);
-      pp &= pp - 1;
-      if (*np < 2) pl[*np] = o;
-      (*np)++;
-    }
-    while (hh) {
-      int o = 64 * w + __ffsll((unsigned long long)hh);
-      hh &= hh - 1;
-      if (*nh < 2) hl[*nh] = o;
-      (*nh)++;
-    }
-  }
-}
-
-template <int NI>
-__device__ inline int spin_nperm(const uint64_t *a, int nh, const int *hl, const int *pl) {
-  int n = 0;
-  for (int l = 0; l < nh; ++l) n += count_between1<NI>(a, hl[l], pl[l]);
-  if (nh == 2) {
-    int A = min(hl[0], pl[0]), B = max(hl[0], pl[0]), Cc = min(hl[1], pl[1]), D = max(hl[1], pl[1]);
+__device__ inline int spin_nperm(const uint64_t *det, int s, const SpinExc &e) {
+  int n = count_between1<NI>(det, s, e.h0, e.p0);
+  if (e.nh == 2) {
+    n += count_between1<NI>(det, s, e.h1, e.p1);
+    int A = min(e.h0, e.p0), B = max(e.h0, e.p0), Cc = min(e.h1, e.p1), D = max(e.h1, e.p1);
     if (A < Cc && Cc < B && B < D) n++;
   }
   return n;
@@ -315,38 +358,38 @@ __device__ double i_H_j(const Tabs &T, const uint64_t *ki, const uint64_t *kj, i
   for (int w = 0; w < 2 * NI; ++w) degree += __popcll(ki[w] ^ kj[w]);
   degree >>= 1;
   if (degree > 2) return 0.0;
-  int nh[2], np[2], hl[2][2], pl[2][2];
-  spin_lists<NI>(ki, kj, &nh[0], hl[0], &np[0], pl[0]);
-  spin_lists<NI>(ki + NI, kj + NI, &nh[1], hl[1], &np[1], pl[1]);
+  const SpinExc ea = spin_exc<NI>(ki, kj, 1), eb = spin_exc<NI>(ki, kj, 2);
   if (degree == 2) {
     int nperm = 0;
-    for (int s = 0; s < 2; ++s) {
-      if (nh[s] == 0) continue;
-      nperm += spin_nperm<NI>(ki + s * NI, nh[s], hl[s], pl[s]);
-      if (nh[s] == 2) break;
-    }
+    if (ea.nh) nperm += spin_nperm<NI>(ki, 1, ea);
+    if (ea.nh != 2 && eb.nh) nperm += spin_nperm<NI>(ki, 2, eb);
     double phase = (nperm & 1) ? -1.0 : 1.0;
-    if (nh[0] == 1) {
-      return phase * I4(T, hl[0][0], hl[1][0], pl[0][0], pl[1][0]);
+    if (ea.nh == 1) {
+      return phase * I4(T, ea.h0, eb.h0, ea.p0, eb.p0);
     }
-    int s = (nh[0] == 2) ? 0 : 1;
-    return phase * (I4(T, hl[s][0], hl[s][1], pl[s][0], pl[s][1]) - I4(T, hl[s][0], hl[s][1], pl[s][1], pl[s][0]));
+    const bool sa = ea.nh == 2;
+    const int h0 = sa ? ea.h0 : eb.h0, h1 = sa ? ea.h1 : eb.h1, p0 = sa ? ea.p0 : eb.p0, p1 = sa ? ea.p1 : eb.p1;
+    return phase * (I4(T, h0, h1, p0, p1) - I4(T, h0, h1, p1, p0));
   }
   if (degree == 1) {
-    int s = (nh[0] == 1) ? 0 : 1;
-    double phase = (spin_nperm<NI>(ki + s * NI, 1, hl[s], pl[s]) & 1) ? -1.0 : 1.0;
+    const int s = (ea.nh == 1) ? 1 : 2;
+    const SpinExc e = (s == 1) ? ea : eb;
+    double phase = (spin_nperm<NI>(ki, s, e) & 1) ? -1.0 : 1.0;
     // get_mono_excitation_from_fock(ki, kj, h:=particle, p:=hole, spin)
-    const int h = pl[s][0], p = hl[s][0];
+    const int h = e.p0, p = e.h0;
     const int M = T.M;
     const int64_t Php = pair_index(h - 1, p - 1);
     double hij = T.fock_cs[(h - 1) + M * (p - 1)];
     // reference closed shell: orbitals 1..nb_e in both spins
     uint64_t refcs[NI];
 #pragma unroll
-    for (int w = 0; w < NI; ++w) refcs[w] = 0;
-    for (int o = 1; o <= nb_e; ++o) refcs[(o - 1) >> 6] |= 1ULL << ((o - 1) & 63);
+    for (int w = 0; w < NI; ++w) {
+      const int n = nb_e - 64 * w;
+      refcs[w] = n <= 0 ? 0ULL : (n >= 64 ? ~0ULL : ((1ULL << n) - 1));
+    }
     auto coul = [&](int i) { return T.V[Php * T.npair + pair_index(i - 1, i - 1)]; };
     auto exch = [&](int i) { return T.V[pair_index(h - 1, i - 1) * T.npair + pair_index(i - 1, p - 1)]; };
+#pragma unroll
     for (int ss = 0; ss < 2; ++ss)
 #pragma unroll
       for (int w = 0; w < NI; ++w) {
@@ -355,9 +398,10 @@ __device__ double i_H_j(const Tabs &T, const uint64_t *ki, const uint64_t *kj, i
       }
 #pragma unroll
     for (int w = 0; w < NI; ++w) {
-      uint64_t y = (ki[s * NI + w] ^ refcs[w]) & refcs[w];
+      uint64_t y = (spin_word<NI>(ki, s, w) ^ refcs[w]) & refcs[w];
       while (y) { int o = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; hij += exch(o); }
     }
+#pragma unroll
     for (int ss = 0; ss < 2; ++ss)
 #pragma unroll
       for (int w = 0; w < NI; ++w) {
@@ -366,7 +410,8 @@ __device__ double i_H_j(const Tabs &T, const uint64_t *ki, const uint64_t *kj, i
       }
 #pragma unroll
     for (int w = 0; w < NI; ++w) {
-      uint64_t y = (ki[s * NI + w] ^ refcs[w]) & ki[s * NI + w];
+      const uint64_t kw = spin_word<NI>(ki, s, w);
+      uint64_t y = (kw ^ refcs[w]) & kw;
       while (y) { int o = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; hij -= exch(o); }
     }
     return hij * phase;
@@ -563,8 +608,8 @@ template <int NI>
 __device__ inline void build_mask(const uint64_t *gen, int s1, int h1, int s2, int h2, uint64_t *mask) {
 #pragma unroll
   for (int w = 0; w < 2 * NI; ++w) mask[w] = gen[w];
-  mask[(s1 - 1) * NI + ((h1 - 1) >> 6)] &= ~(1ULL << ((h1 - 1) & 63));
-  mask[(s2 - 1) * NI + ((h2 - 1) >> 6)] &= ~(1ULL << ((h2 - 1) & 63));
+  clear_orb<NI>(mask, s1, h1);
+  clear_orb<NI>(mask, s2, h2);
 }
 
 // LDS state of one block
@@ -663,21 +708,65 @@ struct D2List {
   }
 };
 
+// The particles (orbitals of a selector outside the block's mask, the first
+// 4 kept) and holes (mask orbitals it lacks, the first 2 kept) of each spin,
+// ascending, 1-based.  One orbital per byte (0-based): a run-time spin or
+// position selects with shifts rather than an array index.
+struct SelL {
+  uint32_t pa, pb, ha, hb;
+  int npa, npb, nha, nhb;
+  __device__ int P(int n, int s) const { return (int)(((s == 1 ? pa : pb) >> (8 * (n - 1))) & 0xffu) + 1; }
+  __device__ int H(int n, int s) const { return (int)(((s == 1 ? ha : hb) >> (8 * (n - 1))) & 0xffu) + 1; }
+  __device__ int np(int s) const { return s == 1 ? npa : npb; }
+};
+
 template <int NI>
-__device__ __forceinline__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const int (*h)[2], const int (*p)[4],
-                                       const int *nh, const int *np, int sp, D2List &out) {
+__device__ inline SelL sel_lists(const uint64_t *mask, const uint64_t *d) {
+  SelL L{0u, 0u, 0u, 0u, 0, 0, 0, 0};
+#pragma unroll
+  for (int s = 1; s <= 2; ++s) {
+    uint32_t pw = 0u, hw = 0u;
+    int np = 0, nh = 0;
+#pragma unroll
+    for (int w = 0; w < NI; ++w) {
+      uint64_t mob = ~spin_word<NI>(mask, s, w) & spin_word<NI>(d, s, w);
+      uint64_t per = spin_word<NI>(mask, s, w) & ~spin_word<NI>(d, s, w);
+      while (mob) {
+        const int o = 64 * w + __ffsll((unsigned long long)mob) - 1;
+        mob &= mob - 1;
+        if (np < 4) pw |= (uint32_t)o << (8 * np);
+        np++;
+      }
+      while (per) {
+        const int o = 64 * w + __ffsll((unsigned long long)per) - 1;
+        per &= per - 1;
+        if (nh < 2) hw |= (uint32_t)o << (8 * nh);
+        nh++;
+      }
+    }
+    if (s == 1) {
+      L.pa = pw; L.ha = hw; L.npa = np; L.nha = nh;
+    } else {
+      L.pb = pw; L.hb = hw; L.npb = np; L.nhb = nh;
+    }
+  }
+  return L;
+}
+
+template <int NI>
+__device__ __forceinline__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const SelL &SL, int sp, D2List &out) {
   uint64_t pmk[2 * NI];
   prefix_parity<NI>(sel, pmk);
-  // p[s][k] = k+1-th particle of spin s+1; h similarly.  The (at most 6)
+  // P(k, s) = k-th particle of spin s; H similarly.  The (at most 6)
   // cells are collected first, their integrals loaded together, then the
   // values pushed in the reference's order: (V[ia] - V[ib]) * sign, or
   // V[ia] * sign when ib < 0.
-#define P(n, s) p[(s) - 1][(n) - 1]
-#define H(n, s) h[(s) - 1][(n) - 1]
-  int tip = np[0] * np[1];
+#define P(n, s) SL.P((n), (s))
+#define H(n, s) SL.H((n), (s))
+  int tip = SL.npa * SL.npb;
   int ma = sp;
-  if (np[0] > np[1]) ma = 1;
-  if (np[0] < np[1]) ma = 2;
+  if (SL.npa > SL.npb) ma = 1;
+  if (SL.npa < SL.npb) ma = 2;
   int mi = (ma % 2) + 1;
   const uint32_t *B1 = C.L.ban1;
   const int M = C.M;
@@ -805,9 +894,8 @@ __device__ __forceinline__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const in
 
 
 template <int NI>
-__device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const int (*h)[2], const int (*p)[4],
-                                       const int *nh, const int *np, int sp, const double *coefs, int na_e,
-                                       int nb_e) {
+__device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const SelL &SL, int sp, const double *coefs,
+                                       int na_e, int nb_e) {
   uint64_t pmk[2 * NI];
   prefix_parity<NI>(sel, pmk);
   const int M = C.M;
@@ -816,12 +904,14 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const in
   // lbanned = bannedOrb + the selector's particles
   auto lbanned = [&](int o, int s) -> bool {
     if (borb_get(BO, o, s)) return true;
-    for (int k = 1; k <= np[s - 1]; ++k)
-      if (P(k, s) == o) return true;
+    const int n = SL.np(s);
+#pragma unroll
+    for (int k = 1; k <= 4; ++k)
+      if (k <= n && P(k, s) == o) return true;
     return false;
   };
   int ma = 1;
-  if (np[1] >= 2) ma = 2;
+  if (SL.npb >= 2) ma = 2;
   int mi = (ma == 1) ? 2 : 1;
   // The cells this thread owns on the selector's lines (at most 6) are
   // collected first, their integrals loaded together, then added: value =
@@ -873,7 +963,7 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const in
       sib[1 + which] = -1;
       sg[1 + which] = phase_pm<NI>(pmk, ma, mi, hfix, other, pu, pfix);
     }
-  } else if (np[ma - 1] == 3) {
+  } else if (SL.np(ma) == 3) {
     const int hfix = H(1, ma);
 #pragma unroll
     for (int i = 1; i <= 3; ++i) {
@@ -962,17 +1052,17 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const in
   // singles folded into this block
   int s1, s2;
   if (sp == 3) { s1 = 1; s2 = 2; } else { s1 = sp; s2 = sp; }
-  for (int i1 = 1; i1 <= np[s1 - 1]; ++i1) {
+  for (int i1 = 1; i1 <= SL.np(s1); ++i1) {
     int ib = (s1 == s2) ? i1 + 1 : 1;
-    for (int i2 = ib; i2 <= np[s2 - 1]; ++i2) {
+    for (int i2 = ib; i2 <= SL.np(s2); ++i2) {
       int p1 = P(i1, s1), p2 = P(i2, s2);
       if (!owns(p1, p2, M)) continue;
       if (borb_get(BO, p1, s1) || borb_get(BO, p2, s2) || ban_get(B1, M, p1, p2)) continue;
       uint64_t det[2 * NI];
 #pragma unroll
       for (int w = 0; w < 2 * NI; ++w) det[w] = C.mask[w];
-      det[(s1 - 1) * NI + ((p1 - 1) >> 6)] |= 1ULL << ((p1 - 1) & 63);
-      det[(s2 - 1) * NI + ((p2 - 1) >> 6)] |= 1ULL << ((p2 - 1) & 63);
+      set_orb<NI>(det, s1, p1);
+      set_orb<NI>(det, s2, p2);
       double hij = i_H_j<NI>(C.T, sel, det, na_e, nb_e);
       C.gath += 4;
       C.add(p1, p2, coefs, hij);
@@ -981,8 +1071,8 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const in
 }
 
 template <int NI>
-__device__ __forceinline__ void get_d0(Ctx<NI> &C, const uint64_t *sel, const int (*p)[4], int sp, const double *coefs, int na_e,
-                       int nb_e) {
+__device__ __forceinline__ void get_d0(Ctx<NI> &C, const uint64_t *sel, const SelL &SL, int sp, const double *coefs,
+                                       int na_e, int nb_e) {
   // one cell per orbital row and thread.  Cells on the rows and columns of
   // the selector's own particles need the full matrix element (i_H_j) and
   // are done separately; for the others the integrals of 8 (N_int = 1) or 4
@@ -996,8 +1086,8 @@ __device__ __forceinline__ void get_d0(Ctx<NI> &C, const uint64_t *sel, const in
     uint64_t det[2 * NI];
 #pragma unroll
     for (int w = 0; w < 2 * NI; ++w) det[w] = C.mask[w];
-    det[(si - 1) * NI + ((i - 1) >> 6)] |= 1ULL << ((i - 1) & 63);
-    det[(sj - 1) * NI + ((j - 1) >> 6)] |= 1ULL << ((j - 1) & 63);
+    set_orb<NI>(det, si, i);
+    set_orb<NI>(det, sj, j);
     C.gath += 4;
     C.add(i, j, coefs, i_H_j<NI>(C.T, sel, det, na_e, nb_e));
   };
@@ -1070,9 +1160,9 @@ __device__ __forceinline__ void get_d0(Ctx<NI> &C, const uint64_t *sel, const in
       if (borb_get(BO, i, sp) || borb_get(BO, j, sp) || ban_get(B1, M, i, j)) return;
       full(i, j, sp, sp);
     };
-    const int q[2] = {p1, p2};
+#pragma unroll
     for (int a = 0; a < 2; ++a) {
-      const int o = q[a];
+      const int o = a == 0 ? p1 : p2;
       // row o: cell (o, j) with j > o
       const int j = owned_j(o, M);
       if (j) try_cell(o, j);
@@ -1085,30 +1175,6 @@ __device__ __forceinline__ void get_d0(Ctx<NI> &C, const uint64_t *sel, const in
 #undef H
 }
 
-template <int NI>
-__device__ inline void sel_lists(const uint64_t *mask, const uint64_t *d, int (*p)[4], int *np, int (*h)[2], int *nh) {
-  for (int s = 0; s < 2; ++s) {
-    np[s] = 0;
-    nh[s] = 0;
-#pragma unroll
-    for (int w = 0; w < NI; ++w) {
-      uint64_t mob = ~mask[s * NI + w] & d[s * NI + w];
-      uint64_t per = mask[s * NI + w] & ~d[s * NI + w];
-      while (mob) {
-        int o = 64 * w + __ffsll((unsigned long long)mob);
-        mob &= mob - 1;
-        if (np[s] < 4) p[s][np[s]] = o;
-        np[s]++;
-      }
-      while (per) {
-        int o = 64 * w + __ffsll((unsigned long long)per);
-        per &= per - 1;
-        if (nh[s] < 2) h[s][nh[s]] = o;
-        nh[s]++;
-      }
-    }
-  }
-}
 
 // diag_H_mat_elem_fock (slater_rules.irp.f:1800-1867)
 __device__ inline double diag_fock(const Tabs &T, const double *f, int sp, int h1, int p1, int h2, int p2) {
@@ -1263,16 +1329,19 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
       atomicOr(&L.flag[0], 1);
       continue;
     }
-    int lst[2], n = 0;
+    int l0 = 0, l1 = 0, n = 0;
 #pragma unroll
     for (int w = 0; w < 2 * NI; ++w) {
       uint64_t y = d[w] & ~mask[w];
       while (y && n < 2) {
-        lst[n++] = 64 * (w % NI) + __ffsll((unsigned long long)y);
+        const int o = 64 * (w % NI) + __ffsll((unsigned long long)y);
+        if (n == 0) l0 = o;
+        else l1 = o;
+        ++n;
         y &= y - 1;
       }
     }
-    if (n == 2) ban_set(L.ban1, M, lst[0], lst[1]);
+    if (n == 2) ban_set(L.ban1, M, l0, l1);
   }
   __syncthreads();
   if (L.flag[0]) {
@@ -1296,22 +1365,23 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
 #pragma unroll
     for (int w = 0; w < 2 * NI; ++w) nt += __popcll(~mask[w] & d[w]);
     if (nt < 3 || nt > 4) continue;
-    int p[2][4], np[2], h[2][2], nh[2];
-    sel_lists<NI>(mask, d, p, np, h, nh);
+    const SelL SL = sel_lists<NI>(mask, d);
     if (nt == 4) {
       if (sp == 3) {
-        for (int i = 0; i < np[0]; ++i)
-          for (int j = 0; j < np[1]; ++j) ban_set(L.ban1, M, p[0][i], p[1][j]);
+        for (int i = 1; i <= SL.npa; ++i)
+          for (int j = 1; j <= SL.npb; ++j) ban_set(L.ban1, M, SL.P(i, 1), SL.P(j, 2));
       } else {
-        for (int i = 0; i < np[sp - 1]; ++i)
-          for (int j = 0; j < i; ++j) {
-            ban_set(L.ban1, M, p[sp - 1][j], p[sp - 1][i]);
-            ban_set(L.ban1, M, p[sp - 1][i], p[sp - 1][j]);
+        const int n = SL.np(sp);
+        for (int i = 1; i <= n; ++i)
+          for (int j = 1; j < i; ++j) {
+            ban_set(L.ban1, M, SL.P(j, sp), SL.P(i, sp));
+            ban_set(L.ban1, M, SL.P(i, sp), SL.P(j, sp));
           }
       }
     } else {
-      for (int s = 0; s < 2; ++s)
-        for (int i = 0; i < np[s]; ++i) borb_set(L.borb, p[s][i], s + 1);
+#pragma unroll
+      for (int s = 1; s <= 2; ++s)
+        for (int i = 1; i <= SL.np(s); ++i) borb_set(L.borb, SL.P(i, s), s);
     }
   }
   __syncthreads();
@@ -1448,10 +1518,9 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
                                               : A.psi + (int64_t)(Q.idx[k] - 1) * 2 * NI;
 #pragma unroll
       for (int w = 0; w < 2 * NI; ++w) d[w] = dp[w];
-      int p[2][4], np[2], h[2][2], nh[2];
-      sel_lists<NI>(mask, d, p, np, h, nh);
-      if (kind == 1) get_d1<NI>(C, d, h, p, nh, np, sp, Q.coef + (size_t)NS * k, A.na, A.nb);
-      else get_d0<NI>(C, d, p, sp, Q.coef + (size_t)NS * k, A.na, A.nb);
+      const SelL SL = sel_lists<NI>(mask, d);
+      if (kind == 1) get_d1<NI>(C, d, SL, sp, Q.coef + (size_t)NS * k, A.na, A.nb);
+      else get_d0<NI>(C, d, SL, sp, Q.coef + (size_t)NS * k, A.na, A.nb);
     }
     apply_until(nq);
   };
@@ -1512,9 +1581,8 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
         C.gath += NS;
         D2List dl{Q.cellw + 3 * e, Q.hij + 6 * e, 0};
         if (nt == 4) {
-          int p[2][4], np[2], h[2][2], nh[2];
-          sel_lists<NI>(mask, d, p, np, h, nh);
-          get_d2<NI>(C, d, h, p, nh, np, sp, dl);
+          const SelL SL = sel_lists<NI>(mask, d);
+          get_d2<NI>(C, d, SL, sp, dl);
         } else if (kQueueHoldsDet<NI>) {
           // three or fewer particles: the determinant goes where the
           // four-particle couplings would be, for the replay to read from LDS
@@ -1541,8 +1609,9 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
 
   // ---- fill_buffer_double ----
   const int s1 = (sp == 3) ? 1 : sp, s2 = (sp == 3) ? 2 : sp;
-  double pt2[MAXNS];
-  for (int st = 0; st < NS; ++st) pt2[st] = 0.0;
+  double pt2[MAXNS];   // indexed with constants only (kept in registers)
+#pragma unroll
+  for (int st = 0; st < MAXNS; ++st) pt2[st] = 0.0;
   int64_t ncand = 0;
   bool overflow = false;
   int r_lo, r_hi;
@@ -1559,7 +1628,9 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
     if (sa == 0.0) continue;
     double Hii = diag_fock(T, L.fock, sp, B.h1, p1, B.h2, p2);
     double min_e = 0.0;
-    for (int st = 0; st < NS; ++st) {
+#pragma unroll
+    for (int st = 0; st < MAXNS; ++st) {
+      if (st >= NS) break;
       double dE = A.E0[st] - Hii;
       double val = m[st] + m[st];
       double tmp = sqrt(dE * dE + val * val);
@@ -1575,8 +1646,8 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
         uint64_t det[2 * NI];
 #pragma unroll
         for (int w = 0; w < 2 * NI; ++w) det[w] = mask[w];
-        det[(s1 - 1) * NI + ((p1 - 1) >> 6)] |= 1ULL << ((p1 - 1) & 63);
-        det[(s2 - 1) * NI + ((p2 - 1) >> 6)] |= 1ULL << ((p2 - 1) & 63);
+        set_orb<NI>(det, s1, p1);
+        set_orb<NI>(det, s2, p2);
         A.pool_key[slot] = min_e;
 #pragma unroll
         for (int w = 0; w < 2 * NI; ++w) A.pool_det[slot * 2 * NI + w] = det[w];
@@ -1591,7 +1662,9 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
   if (tid == 0) L.flag[1] = 0;
   __syncthreads();
   if (overflow) atomicOr(&L.flag[1], 1);
-  for (int st = 0; st < NS; ++st) red[st * SB + tid] = pt2[st];
+#pragma unroll
+  for (int st = 0; st < MAXNS; ++st)
+    if (st < NS) red[st * SB + tid] = pt2[st];
   __syncthreads();
   for (int off = SB / 2; off > 0; off >>= 1) {
     if (tid < off)
