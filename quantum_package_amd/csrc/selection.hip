@@ -151,14 +151,14 @@ __device__ inline void prefix_parity(const uint64_t *sel, uint64_t *pmk) {
 template <int NI>
 __device__ inline int pm_bit(const uint64_t *pmk, int s, int o) { return (int)((orb_word<NI>(pmk, s, o) >> ((o - 1) & 63)) & 1); }
 
-// phase_bi from the prefix-parity masks
+// phase_bi from the prefix-parity masks: 1 when the phase is -1
 template <int NI>
-__device__ inline double phase_pm(const uint64_t *pmk, int s1, int s2, int h1, int p1, int h2, int p2) {
+__device__ inline uint32_t odd_pm(const uint64_t *pmk, int s1, int s2, int h1, int p1, int h2, int p2) {
   int np = pm_bit<NI>(pmk, s1, h1) + pm_bit<NI>(pmk, s1, p1) + pm_bit<NI>(pmk, s2, h2) + pm_bit<NI>(pmk, s2, p2);
   if (p1 < h1) np++;
   if (p2 < h2) np++;
   if (s1 == s2 && max(h1, p1) > min(h2, p2)) np++;
-  return (np & 1) ? -1.0 : 1.0;
+  return (uint32_t)(np & 1);
 }
 
 struct Tabs {
@@ -696,17 +696,61 @@ struct Ctx {
 // written as (cell, value) pairs in the reference's order straight into the
 // selector's LDS queue entry.
 struct D2List {
-  uint32_t *cellw;   // 3 words, two cells i | j << 8 (1-based) per word
+  uint32_t *cellw;   // 3 words, two cells (i - 1) | (j - 1) << 8 per word
   double *hij;
   int n;
-  __device__ inline void push(int i, int j, double v) {
-    const uint32_t c = (uint32_t)i | ((uint32_t)j << 8);
+  __device__ inline void push(uint32_t c, double v) {   // c = (i - 1) | (j - 1) << 8
     if (n & 1) cellw[n >> 1] |= c << 16;
     else cellw[n >> 1] = c;
     hij[n] = v;
     ++n;
   }
 };
+
+// Up to 6 cells of one selector, collected before their integrals are
+// loaded together: cell (i - 1) | (j - 1) << 8, table offsets of the value's
+// one or two integrals (offsets fit 32 bits: M <= 256 for V, the M^4 table
+// stops at kG4MaxOrb), and flags: bit k, value va - vb (else va); bit 8 + k,
+// negative phase; bit 16 + k, slot used.  The value is (va - vb) * sign or
+// va * sign with sign = +-1, formed as an exact negation.
+struct CellSet {
+  uint32_t cl[6], ia[6], ib[6], fl;
+  __device__ inline void clear() {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) cl[k] = ia[k] = ib[k] = 0u;
+    fl = 0u;
+  }
+  __device__ inline bool used(int k) const { return (fl >> (16 + k)) & 1u; }
+  __device__ inline void put2(int k, int i, int j, int64_t a, int64_t b, uint32_t odd) {
+    cl[k] = (uint32_t)(i - 1) | ((uint32_t)(j - 1) << 8);
+    ia[k] = (uint32_t)a;
+    ib[k] = (uint32_t)b;
+    fl |= (1u << k) | (odd << (8 + k)) | (1u << (16 + k));
+  }
+  __device__ inline void put1(int k, int i, int j, int64_t a, uint32_t odd) {
+    cl[k] = (uint32_t)(i - 1) | ((uint32_t)(j - 1) << 8);
+    ia[k] = (uint32_t)a;
+    fl |= (odd << (8 + k)) | (1u << (16 + k));
+  }
+  // loads the integrals of the used cells and forms their values
+  __device__ inline void values(const double *TT, double *v, int64_t &gath) const {
+    double va[6], vb[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      va[k] = used(k) ? TT[ia[k]] : 0.0;
+      vb[k] = ((fl >> k) & 1u) ? TT[ib[k]] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      double x = ((fl >> k) & 1u) ? va[k] - vb[k] : va[k];
+      if ((fl >> (8 + k)) & 1u) x = -x;
+      v[k] = x;
+      if (used(k)) gath += ((fl >> k) & 1u) ? 2 : 1;
+    }
+  }
+};
+__device__ inline int cell_i(uint32_t c) { return (int)(c & 0xffu) + 1; }
+__device__ inline int cell_j(uint32_t c) { return (int)((c >> 8) & 0xffu) + 1; }
 
 // The particles (orbitals of a selector outside the block's mask, the first
 // 4 kept) and holes (mask orbitals it lacks, the first 2 kept) of each spin,
@@ -759,8 +803,7 @@ __device__ __forceinline__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const Se
   prefix_parity<NI>(sel, pmk);
   // P(k, s) = k-th particle of spin s; H similarly.  The (at most 6)
   // cells are collected first, their integrals loaded together, then the
-  // values pushed in the reference's order: (V[ia] - V[ib]) * sign, or
-  // V[ia] * sign when ib < 0.
+  // values pushed in the reference's order.
 #define P(n, s) SL.P((n), (s))
 #define H(n, s) SL.H((n), (s))
   int tip = SL.npa * SL.npb;
@@ -770,12 +813,8 @@ __device__ __forceinline__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const Se
   int mi = (ma % 2) + 1;
   const uint32_t *B1 = C.L.ban1;
   const int M = C.M;
-  bool sv[6];
-  int sci[6], scj[6];
-  int64_t sia[6], sib[6];
-  double sg[6];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) sv[k] = false;
+  CellSet cs;
+  cs.clear();
   auto idx4 = [&](int i, int j, int k, int l) -> int64_t {   // <ij|kl>, 1-based
     return pair_index(i - 1, k - 1) * (int64_t)C.T.npair + pair_index(j - 1, l - 1);
   };
@@ -790,12 +829,8 @@ __device__ __forceinline__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const Se
         const int i1 = (i == 1) ? 2 : 1, i2 = (i == 3) ? 2 : 3;
         const int p1 = P(i1, ma), p2 = P(i2, ma), h1 = H(1, ma), h2 = H(2, ma);
         const int k = i - 1;
-        sv[k] = true;
-        sci[k] = ma == 1 ? putj : puti;
-        scj[k] = ma == 1 ? puti : putj;
-        sia[k] = idx4(p1, p2, h1, h2);
-        sib[k] = idx4(p2, p1, h1, h2);
-        sg[k] = phase_pm<NI>(pmk, ma, ma, h1, p1, h2, p2);
+        cs.put2(k, ma == 1 ? putj : puti, ma == 1 ? puti : putj, idx4(p1, p2, h1, h2), idx4(p2, p1, h1, h2),
+                odd_pm<NI>(pmk, ma, ma, h1, p1, h2, p2));
       }
     } else {
       const int h1 = H(1, 1), h2 = H(1, 2);
@@ -808,12 +843,7 @@ __device__ __forceinline__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const Se
           if (ban_get(BB, M, puti, putj)) continue;
           const int p1 = P(3 - i, 1);
           const int k = 2 * (j - 1) + (i - 1);
-          sv[k] = true;
-          sci[k] = puti;
-          scj[k] = putj;
-          sia[k] = idx4(p1, p2, h1, h2);
-          sib[k] = -1;
-          sg[k] = phase_pm<NI>(pmk, 1, 2, h1, p1, h2, p2);
+          cs.put1(k, puti, putj, idx4(p1, p2, h1, h2), odd_pm<NI>(pmk, 1, 2, h1, p1, h2, p2));
         }
       }
     }
@@ -834,12 +864,7 @@ __device__ __forceinline__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const Se
           for (int q = 1; q <= 4; ++q)
             if (q != i && q != j) c[n++] = q;
           const int p1 = P(c[0], ma), p2 = P(c[1], ma);
-          sv[kk] = true;
-          sci[kk] = puti;
-          scj[kk] = putj;
-          sia[kk] = idx4(p1, p2, h1, h2);
-          sib[kk] = idx4(p2, p1, h1, h2);
-          sg[kk] = phase_pm<NI>(pmk, ma, ma, h1, p1, h2, p2);
+          cs.put2(kk, puti, putj, idx4(p1, p2, h1, h2), idx4(p2, p1, h1, h2), odd_pm<NI>(pmk, ma, ma, h1, p1, h2, p2));
         }
       }
     } else if (tip == 3) {
@@ -851,45 +876,21 @@ __device__ __forceinline__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const Se
         if (ban_get(B1, M, puti, putj)) continue;
         const int p2 = P(i, ma);
         const int k = i - 1;
-        sv[k] = true;
-        sci[k] = min(puti, putj);
-        scj[k] = max(puti, putj);
-        sia[k] = idx4(p1, p2, h1, h2);
-        sib[k] = -1;
-        sg[k] = phase_pm<NI>(pmk, mi, ma, h1, p1, h2, p2);
+        cs.put1(k, min(puti, putj), max(puti, putj), idx4(p1, p2, h1, h2), odd_pm<NI>(pmk, mi, ma, h1, p1, h2, p2));
       }
     } else {
       const int puti = P(1, sp), putj = P(2, sp);
       if (!ban_get(B1, M, puti, putj)) {
         const int p1 = P(1, mi), p2 = P(2, mi), h1 = H(1, mi), h2 = H(2, mi);
-        sv[0] = true;
-        sci[0] = puti;
-        scj[0] = putj;
-        sia[0] = idx4(p1, p2, h1, h2);
-        sib[0] = idx4(p2, p1, h1, h2);
-        sg[0] = phase_pm<NI>(pmk, mi, mi, h1, p1, h2, p2);
+        cs.put2(0, puti, putj, idx4(p1, p2, h1, h2), idx4(p2, p1, h1, h2), odd_pm<NI>(pmk, mi, mi, h1, p1, h2, p2));
       }
     }
   }
-  double va[6], vb[6];
+  double v[6];
+  cs.values(C.T.V, v, C.gath);
 #pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    va[k] = sv[k] ? C.T.V[sia[k]] : 0.0;
-    vb[k] = (sv[k] && sib[k] >= 0) ? C.T.V[sib[k]] : 0.0;
-  }
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    if (!sv[k]) continue;
-    double hij;
-    if (sib[k] >= 0) {
-      C.gath += 2;
-      hij = (va[k] - vb[k]) * sg[k];
-    } else {
-      C.gath += 1;
-      hij = va[k] * sg[k];
-    }
-    out.push(sci[k], scj[k], hij);
-  }
+  for (int k = 0; k < 6; ++k)
+    if (cs.used(k)) out.push(cs.cl[k], v[k]);
 }
 
 
@@ -914,14 +915,9 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const Se
   if (SL.npb >= 2) ma = 2;
   int mi = (ma == 1) ? 2 : 1;
   // The cells this thread owns on the selector's lines (at most 6) are
-  // collected first, their integrals loaded together, then added: value =
-  // (V[ia] - V[ib]) * sign, or V[ia] * sign when ib < 0.
-  bool sv[6];
-  int sci[6], scj[6];
-  int64_t sia[6], sib[6];
-  double sg[6];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) sv[k] = false;
+  // collected first, their integrals loaded together, then added.
+  CellSet cs;
+  cs.clear();
   // <ij|kl> (1-based) with argument `vary` the orbital that changes along
   // the line (the cells' owners are consecutive threads)
   auto idx4 = [&](int i, int j, int k, int l, int vary) -> int64_t { return line_off(C.T, i, j, k, l, vary); };
@@ -933,18 +929,13 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const Se
     if (!borb_get(BO, puti, mi)) {
       const int putj = (ma == 1) ? row_of(puti) : owned_j(puti, M);
       if (putj && putj != hfix && !lbanned(putj, ma) && !ban_get(BB, M, putj, puti)) {
-        sv[0] = true;
-        sci[0] = (ma == 1) ? putj : puti;
-        scj[0] = (ma == 1) ? puti : putj;
-        if (putj < hfix) {
-          sia[0] = idx4(p1, p2, putj, hfix, 2);
-          sib[0] = idx4(p2, p1, putj, hfix, 2);
-          sg[0] = phase_pm<NI>(pmk, ma, ma, putj, p1, hfix, p2);
-        } else {
-          sia[0] = idx4(p1, p2, hfix, putj, 3);
-          sib[0] = idx4(p2, p1, hfix, putj, 3);
-          sg[0] = phase_pm<NI>(pmk, ma, ma, hfix, p1, putj, p2);
-        }
+        const int ci = (ma == 1) ? putj : puti, cj = (ma == 1) ? puti : putj;
+        if (putj < hfix)
+          cs.put2(0, ci, cj, idx4(p1, p2, putj, hfix, 2), idx4(p2, p1, putj, hfix, 2),
+                  odd_pm<NI>(pmk, ma, ma, putj, p1, hfix, p2));
+        else
+          cs.put2(0, ci, cj, idx4(p1, p2, hfix, putj, 3), idx4(p2, p1, hfix, putj, 3),
+                  odd_pm<NI>(pmk, ma, ma, hfix, p1, putj, p2));
       }
     }
     const int pfix = P(1, mi);
@@ -956,12 +947,8 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const Se
       const int pu = (mi == 1) ? row_of(pf) : owned_j(pf, M);
       if (!pu || lbanned(pu, mi)) continue;
       if (ban_get(BB, M, pf, pu)) continue;
-      sv[1 + which] = true;
-      sci[1 + which] = (mi == 1) ? pu : pf;
-      scj[1 + which] = (mi == 1) ? pf : pu;
-      sia[1 + which] = idx4(other, pfix, hfix, pu, 3);
-      sib[1 + which] = -1;
-      sg[1 + which] = phase_pm<NI>(pmk, ma, mi, hfix, other, pu, pfix);
+      cs.put1(1 + which, (mi == 1) ? pu : pf, (mi == 1) ? pf : pu, idx4(other, pfix, hfix, pu, 3),
+              odd_pm<NI>(pmk, ma, mi, hfix, other, pu, pfix));
     }
   } else if (SL.np(ma) == 3) {
     const int hfix = H(1, ma);
@@ -985,18 +972,13 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const Se
         const int putj = x;
         if (putj == hfix) continue;
         if (lbanned(putj, ma) || ban_get(B1, M, puti, putj)) continue;
-        sv[k] = true;
-        sci[k] = half == 0 ? x : puti;
-        scj[k] = half == 0 ? puti : x;
-        if (putj < hfix) {
-          sia[k] = idx4(p1, p2, putj, hfix, 2);
-          sib[k] = idx4(p2, p1, putj, hfix, 2);
-          sg[k] = phase_pm<NI>(pmk, ma, ma, putj, p1, hfix, p2);
-        } else {
-          sia[k] = idx4(p1, p2, hfix, putj, 3);
-          sib[k] = idx4(p2, p1, hfix, putj, 3);
-          sg[k] = phase_pm<NI>(pmk, ma, ma, hfix, p1, putj, p2);
-        }
+        const int ci = half == 0 ? x : puti, cj = half == 0 ? puti : x;
+        if (putj < hfix)
+          cs.put2(k, ci, cj, idx4(p1, p2, putj, hfix, 2), idx4(p2, p1, putj, hfix, 2),
+                  odd_pm<NI>(pmk, ma, ma, putj, p1, hfix, p2));
+        else
+          cs.put2(k, ci, cj, idx4(p1, p2, hfix, putj, 3), idx4(p2, p1, hfix, putj, 3),
+                  odd_pm<NI>(pmk, ma, ma, hfix, p1, putj, p2));
       }
     }
   } else {
@@ -1019,35 +1001,17 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const Se
         const int puti = x;
         if (lbanned(puti, ma)) continue;
         if (ban_get(B1, M, puti, pc)) continue;
-        sv[k] = true;
-        sci[k] = half == 0 ? x : pc;
-        scj[k] = half == 0 ? pc : x;
-        sia[k] = idx4(pfix, other, hfix, puti, 3);
-        sib[k] = -1;
-        sg[k] = phase_pm<NI>(pmk, mi, ma, hfix, pfix, puti, other);
+        cs.put1(k, half == 0 ? x : pc, half == 0 ? pc : x, idx4(pfix, other, hfix, puti, 3),
+                odd_pm<NI>(pmk, mi, ma, hfix, pfix, puti, other));
       }
     }
   }
   {
-    double va[6], vb[6];
+    double v[6];
+    cs.values(TT, v, C.gath);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      va[k] = sv[k] ? TT[sia[k]] : 0.0;
-      vb[k] = (sv[k] && sib[k] >= 0) ? TT[sib[k]] : 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      if (!sv[k]) continue;
-      double hij;
-      if (sib[k] >= 0) {
-        C.gath += 2;
-        hij = (va[k] - vb[k]) * sg[k];
-      } else {
-        C.gath += 1;
-        hij = va[k] * sg[k];
-      }
-      C.add(sci[k], scj[k], coefs, hij);
-    }
+    for (int k = 0; k < 6; ++k)
+      if (cs.used(k)) C.add(cell_i(cs.cl[k]), cell_j(cs.cl[k]), coefs, v[k]);
   }
   // singles folded into this block
   int s1, s2;
@@ -1221,7 +1185,7 @@ __device__ inline double diag_fock(const Tabs &T, const double *f, int sp, int h
 // The LDS queue of a block's surviving selectors, as arrays (qcap entries):
 // per entry its index, kind (0: four particles outside the mask, couplings
 // precomputed; 1: three; 2: two or fewer) with the number of precomputed
-// couplings, the couplings' cells (i | j << 8, two per word) and values, and
+// couplings, the couplings' cells ((i - 1) | (j - 1) << 8, two per word) and values, and
 // its coefficients.  The replay of a flush keeps, per owner thread, the list
 // of (entry, coupling) records of the four-particle entries that fall on
 // its cells (rec, cursor), and the positions of the other entries (k12).
@@ -1443,7 +1407,7 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
           const int nc = kn >> 4;
           for (int c = 0; c < nc; ++c) {
             const uint32_t cell = cell_of(e, c);
-            atomicAdd(&Q.cursor[owner_of((int)(cell & 0xff), (int)(cell >> 8), M)], 1);
+            atomicAdd(&Q.cursor[owner_of(cell_i(cell), cell_j(cell), M)], 1);
           }
         } else {
           is12 = true;
@@ -1482,7 +1446,7 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
       const int nc = kn >> 4;
       for (int c = 0; c < nc; ++c) {
         const uint32_t cell = cell_of(e, c);
-        const int slot = atomicAdd(&Q.cursor[owner_of((int)(cell & 0xff), (int)(cell >> 8), M)], 1);
+        const int slot = atomicAdd(&Q.cursor[owner_of(cell_i(cell), cell_j(cell), M)], 1);
         Q.rec[slot] = (uint16_t)((e << 3) | c);
       }
     }
@@ -1505,7 +1469,7 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
         if (e >= k) break;
         const int c = x & 7;
         const uint32_t cell = cell_of(e, c);
-        C.add((int)(cell & 0xff), (int)(cell >> 8), Q.coef + (size_t)NS * e, Q.hij[6 * e + c]);
+        C.add(cell_i(cell), cell_j(cell), Q.coef + (size_t)NS * e, Q.hij[6 * e + c]);
         ++r;
       }
     };
