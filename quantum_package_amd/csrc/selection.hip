@@ -18,10 +18,10 @@
 //     thread 256 * band(i) + (37 (i-1) + (j-1)) mod 256, so every row and
 //     every column of cells maps to distinct threads;
 //   * the block's selectors are filtered in parallel into an LDS queue in
-//     near-list order.  The filtering thread computes the <= 6 couplings of
-//     a four-particle selector (get_d2) into its queue entry and stores the
-//     determinant of any other survivor.  The replay turns the four-particle
-//     couplings into per-owner record lists (count, scan, scatter, sort);
+//     near-list order, with their determinants.  At a flush, the <= 6
+//     couplings of each four-particle selector (get_d2) are computed into
+//     its queue entry, one entry per thread; the replay then turns them
+//     into per-owner record lists (count, scan, scatter, sort);
 //     each thread then walks the get_d1 / get_d0 entries in queue order and
 //     applies its earlier records before each one, so every cell receives
 //     its terms in selector order, the same order as the reference
@@ -1368,12 +1368,12 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
 
   // ---- splash_pq: couplings in selector order ----
   // Phase A: SB near-list entries at a time are filtered in parallel and the
-  // survivors appended, in near-list order, to an LDS queue; the filtering
-  // thread also computes the couplings of a four-particle selector
-  // (get_d2).  Phase B (flush): the queue in order; a four-particle entry
-  // adds its precomputed values in the cells' owner threads, the others run
-  // get_d1 / get_d0 with every thread.  Each cell thus receives its terms in
-  // selector order, as in the reference.
+  // survivors appended, in near-list order, to an LDS queue with their
+  // determinants.  Phase B (flush): the couplings of the four-particle
+  // entries are computed, one entry per thread (get_d2); then the queue in
+  // order: a four-particle entry adds its values in the cells' owner
+  // threads, the others run get_d1 / get_d0 with every thread.  Each cell
+  // thus receives its terms in selector order, as in the reference.
   Ctx<NI> C(T, A, L, mask);
   QueueLds Q;
   Q.carve(reinterpret_cast<unsigned char *>(red), A.qcap, NS);
@@ -1395,6 +1395,21 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
     return (Q.cellw[3 * e + (c >> 1)] >> (16 * (c & 1))) & 0xffffu;
   };
   auto flush = [&](int nq) {
+    // couplings of the four-particle entries (get_d2), one entry per thread:
+    // the queue is dense, so the lanes are busy where the filter's survivors
+    // were scattered over its chunk
+    for (int e = tid; e < nq; e += SB) {
+      if (Q.kind_n[e] != 0) continue;
+      uint64_t d[2 * NI];
+      const uint64_t *dp = kQueueHoldsDet<NI> ? reinterpret_cast<const uint64_t *>(Q.hij + 6 * e)
+                                              : A.psi + (int64_t)(Q.idx[e] - 1) * 2 * NI;
+#pragma unroll
+      for (int w = 0; w < 2 * NI; ++w) d[w] = dp[w];
+      D2List dl{Q.cellw + 3 * e, Q.hij + 6 * e, 0};
+      const SelL SL = sel_lists<NI>(mask, d);
+      get_d2<NI>(C, d, SL, sp, dl);
+      Q.kind_n[e] = dl.n << 4;
+    }
     Q.cursor[tid] = 0;
     __syncthreads();
     int n12 = 0;
@@ -1543,18 +1558,15 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
         Q.idx[e] = idx;
         for (int st = 0; st < NS; ++st) Q.coef[(size_t)NS * e + st] = A.coef[(idx - 1) + A.n_det * st];
         C.gath += NS;
-        D2List dl{Q.cellw + 3 * e, Q.hij + 6 * e, 0};
-        if (nt == 4) {
-          const SelL SL = sel_lists<NI>(mask, d);
-          get_d2<NI>(C, d, SL, sp, dl);
-        } else if (kQueueHoldsDet<NI>) {
-          // three or fewer particles: the determinant goes where the
-          // four-particle couplings would be, for the replay to read from LDS
+        if (kQueueHoldsDet<NI>) {
+          // the determinant goes where a four-particle selector's couplings
+          // will be, for the flush to read from LDS
           uint64_t *qd = reinterpret_cast<uint64_t *>(Q.hij + 6 * e);
 #pragma unroll
           for (int w = 0; w < 2 * NI; ++w) qd[w] = d[w];
         }
-        Q.kind_n[e] = (nt == 4 ? 0 : (nt == 3 ? 1 : 2)) | (dl.n << 4);
+        // kind 0 (four particles): couplings computed at the flush
+        Q.kind_n[e] = nt == 4 ? 0 : (nt == 3 ? 1 : 2);
       }
       const int adv = min(cnt - done, space);
       qn += adv;
