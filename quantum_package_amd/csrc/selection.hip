@@ -389,30 +389,42 @@ __device__ double i_H_j(const Tabs &T, const uint64_t *ki, const uint64_t *kj, i
     }
     auto coul = [&](int i) { return T.V[Php * T.npair + pair_index(i - 1, i - 1)]; };
     auto exch = [&](int i) { return T.V[pair_index(h - 1, i - 1) * T.npair + pair_index(i - 1, p - 1)]; };
+    // hij -/+= f(o) over the set bits o of y in ascending order; the
+    // integrals of four orbitals at a time are loaded together, the sum
+    // keeps its order
+    auto acc = [&](uint64_t y, int w, bool sub, auto f) {
+      while (y) {
+        int o[4];
+        double v[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          o[t] = 0;
+          if (y) {
+            o[t] = 64 * w + __ffsll((unsigned long long)y);
+            y &= y - 1;
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) v[t] = o[t] ? f(o[t]) : 0.0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          if (o[t]) hij = sub ? hij - v[t] : hij + v[t];
+      }
+    };
 #pragma unroll
     for (int ss = 0; ss < 2; ++ss)
 #pragma unroll
-      for (int w = 0; w < NI; ++w) {
-        uint64_t y = (ki[ss * NI + w] ^ refcs[w]) & refcs[w];
-        while (y) { int o = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; hij -= coul(o); }
-      }
+      for (int w = 0; w < NI; ++w) acc((ki[ss * NI + w] ^ refcs[w]) & refcs[w], w, true, coul);
 #pragma unroll
-    for (int w = 0; w < NI; ++w) {
-      uint64_t y = (spin_word<NI>(ki, s, w) ^ refcs[w]) & refcs[w];
-      while (y) { int o = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; hij += exch(o); }
-    }
+    for (int w = 0; w < NI; ++w) acc((spin_word<NI>(ki, s, w) ^ refcs[w]) & refcs[w], w, false, exch);
 #pragma unroll
     for (int ss = 0; ss < 2; ++ss)
 #pragma unroll
-      for (int w = 0; w < NI; ++w) {
-        uint64_t y = (ki[ss * NI + w] ^ refcs[w]) & ki[ss * NI + w];
-        while (y) { int o = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; hij += coul(o); }
-      }
+      for (int w = 0; w < NI; ++w) acc((ki[ss * NI + w] ^ refcs[w]) & ki[ss * NI + w], w, false, coul);
 #pragma unroll
     for (int w = 0; w < NI; ++w) {
       const uint64_t kw = spin_word<NI>(ki, s, w);
-      uint64_t y = (kw ^ refcs[w]) & kw;
-      while (y) { int o = 64 * w + __ffsll((unsigned long long)y); y &= y - 1; hij -= exch(o); }
+      acc((kw ^ refcs[w]) & kw, w, true, exch);
     }
     return hij * phase;
   }
@@ -896,7 +908,7 @@ __device__ __forceinline__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const Se
 
 template <int NI>
 __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const SelL &SL, int sp, const double *coefs,
-                                       int na_e, int nb_e) {
+                                       int na_e, int nb_e, bool singles) {
   uint64_t pmk[2 * NI];
   prefix_parity<NI>(sel, pmk);
   const int M = C.M;
@@ -1013,7 +1025,9 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const Se
     for (int k = 0; k < 6; ++k)
       if (cs.used(k)) C.add(cell_i(cs.cl[k]), cell_j(cs.cl[k]), coefs, v[k]);
   }
-  // singles folded into this block
+  // singles folded into this block (normally applied as records of the
+  // selector's queue entry instead)
+  if (!singles) return;
   int s1, s2;
   if (sp == 3) { s1 = 1; s2 = 2; } else { s1 = sp; s2 = sp; }
   for (int i1 = 1; i1 <= SL.np(s1); ++i1) {
@@ -1398,17 +1412,52 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
     // couplings of the four-particle entries (get_d2), one entry per thread:
     // the queue is dense, so the lanes are busy where the filter's survivors
     // were scattered over its chunk
+    // The same pass computes the single excitations a three-particle
+    // selector folds into the block (the end of get_d1) as records of its
+    // entry, in the slots after its determinant; when they do not fit there
+    // the replay computes them (flag bit 12).  kind_n = kind | records << 4
+    // | first record slot << 8.
     for (int e = tid; e < nq; e += SB) {
-      if (Q.kind_n[e] != 0) continue;
+      const int kind = Q.kind_n[e];
+      if (kind == 2) continue;
       uint64_t d[2 * NI];
       const uint64_t *dp = kQueueHoldsDet<NI> ? reinterpret_cast<const uint64_t *>(Q.hij + 6 * e)
                                               : A.psi + (int64_t)(Q.idx[e] - 1) * 2 * NI;
 #pragma unroll
       for (int w = 0; w < 2 * NI; ++w) d[w] = dp[w];
-      D2List dl{Q.cellw + 3 * e, Q.hij + 6 * e, 0};
       const SelL SL = sel_lists<NI>(mask, d);
-      get_d2<NI>(C, d, SL, sp, dl);
-      Q.kind_n[e] = dl.n << 4;
+      if (kind == 0) {
+        D2List dl{Q.cellw + 3 * e, Q.hij + 6 * e, 0};
+        get_d2<NI>(C, d, SL, sp, dl);
+        Q.kind_n[e] = dl.n << 4;
+      } else {
+        constexpr int c0 = kQueueHoldsDet<NI> ? 2 * NI : 0;
+        const int s1 = (sp == 3) ? 1 : sp, s2 = (sp == 3) ? 2 : sp;
+        auto single_ok = [&](int p1, int p2) {
+          return !(borb_get(L.borb, p1, s1) || borb_get(L.borb, p2, s2) || ban_get(L.ban1, M, p1, p2));
+        };
+        int ns = 0;
+        for (int i1 = 1; i1 <= SL.np(s1); ++i1)
+          for (int i2 = (s1 == s2) ? i1 + 1 : 1; i2 <= SL.np(s2); ++i2) ns += single_ok(SL.P(i1, s1), SL.P(i2, s2));
+        if (c0 + ns > 6) {
+          Q.kind_n[e] = 1 | (1 << 12);
+        } else {
+          D2List dl{Q.cellw + 3 * e, Q.hij + 6 * e, c0};
+          for (int i1 = 1; i1 <= SL.np(s1); ++i1)
+            for (int i2 = (s1 == s2) ? i1 + 1 : 1; i2 <= SL.np(s2); ++i2) {
+              const int p1 = SL.P(i1, s1), p2 = SL.P(i2, s2);
+              if (!single_ok(p1, p2)) continue;
+              uint64_t det[2 * NI];
+#pragma unroll
+              for (int w = 0; w < 2 * NI; ++w) det[w] = mask[w];
+              set_orb<NI>(det, s1, p1);
+              set_orb<NI>(det, s2, p2);
+              C.gath += 4;
+              dl.push((uint32_t)(p1 - 1) | ((uint32_t)(p2 - 1) << 8), i_H_j<NI>(T, d, det, A.na, A.nb));
+            }
+          Q.kind_n[e] = 1 | (ns << 4) | (c0 << 8);
+        }
+      }
     }
     Q.cursor[tid] = 0;
     __syncthreads();
@@ -1418,15 +1467,12 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
       bool is12 = false;
       if (e < nq) {
         const int kn = Q.kind_n[e];
-        if ((kn & 0xf) == 0) {
-          const int nc = kn >> 4;
-          for (int c = 0; c < nc; ++c) {
-            const uint32_t cell = cell_of(e, c);
-            atomicAdd(&Q.cursor[owner_of(cell_i(cell), cell_j(cell), M)], 1);
-          }
-        } else {
-          is12 = true;
+        const int c0 = (kn >> 8) & 7, c1 = c0 + ((kn >> 4) & 0xf);
+        for (int c = c0; c < c1; ++c) {
+          const uint32_t cell = cell_of(e, c);
+          atomicAdd(&Q.cursor[owner_of(cell_i(cell), cell_j(cell), M)], 1);
         }
+        is12 = (kn & 0xf) != 0;
       }
       const unsigned long long bal = __ballot(is12);
       const int before = __popcll(bal & ((1ULL << lane) - 1ULL));
@@ -1457,9 +1503,8 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
     __syncthreads();
     for (int e = tid; e < nq; e += SB) {
       const int kn = Q.kind_n[e];
-      if ((kn & 0xf) != 0) continue;
-      const int nc = kn >> 4;
-      for (int c = 0; c < nc; ++c) {
+      const int c0 = (kn >> 8) & 7, c1 = c0 + ((kn >> 4) & 0xf);
+      for (int c = c0; c < c1; ++c) {
         const uint32_t cell = cell_of(e, c);
         const int slot = atomicAdd(&Q.cursor[owner_of(cell_i(cell), cell_j(cell), M)], 1);
         Q.rec[slot] = (uint16_t)((e << 3) | c);
@@ -1491,14 +1536,15 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
     for (int q = 0; q < n12; ++q) {
       const int k = Q.k12[q];
       apply_until(k);
-      const int kind = Q.kind_n[k] & 0xf;
+      const int kn = Q.kind_n[k];
+      const int kind = kn & 0xf;
       uint64_t d[2 * NI];
       const uint64_t *dp = kQueueHoldsDet<NI> ? reinterpret_cast<const uint64_t *>(Q.hij + 6 * k)
                                               : A.psi + (int64_t)(Q.idx[k] - 1) * 2 * NI;
 #pragma unroll
       for (int w = 0; w < 2 * NI; ++w) d[w] = dp[w];
       const SelL SL = sel_lists<NI>(mask, d);
-      if (kind == 1) get_d1<NI>(C, d, SL, sp, Q.coef + (size_t)NS * k, A.na, A.nb);
+      if (kind == 1) get_d1<NI>(C, d, SL, sp, Q.coef + (size_t)NS * k, A.na, A.nb, (kn >> 12) & 1);
       else get_d0<NI>(C, d, SL, sp, Q.coef + (size_t)NS * k, A.na, A.nb);
     }
     apply_until(nq);
