@@ -176,8 +176,14 @@ struct Tabs {
 // kernels read: in G4 with argument `vary` (0..3 = i, j, k, l) in the
 // contiguous slot, by the 8-fold symmetry (ik|jl) = (ki|jl) = (jl|ik) =
 // (lj|ik); in V (pair matrix) when there is no G4.
-__device__ inline int64_t line_off(const Tabs &T, int i, int j, int k, int l, int vary) {
-  if (!T.G4) return pair_index(i - 1, k - 1) * (int64_t)T.npair + pair_index(j - 1, l - 1);
+// 32-bit offsets: npair^2 < 2^31 for M <= 256, and the M^4 table stops at
+// kG4MaxOrb (160^4 < 2^30); every factor stays below 2^24
+__device__ inline uint32_t pair32(int p, int q) {
+  const int lo = p < q ? p : q, hi = p < q ? q : p;
+  return (uint32_t)(hi * (hi + 1) / 2 + lo);
+}
+__device__ inline uint32_t line_off(const Tabs &T, int i, int j, int k, int l, int vary) {
+  if (!T.G4) return pair32(i - 1, k - 1) * (uint32_t)T.npair + pair32(j - 1, l - 1);
   int a, x, b, y;   // (a x | b y)
   switch (vary) {
     case 0: a = k; x = i; b = j; y = l; break;
@@ -185,8 +191,8 @@ __device__ inline int64_t line_off(const Tabs &T, int i, int j, int k, int l, in
     case 2: a = i; x = k; b = j; y = l; break;
     default: a = j; x = l; b = i; y = k; break;
   }
-  const int M = T.M;
-  return (((int64_t)(a - 1) * M + (b - 1)) * M + (y - 1)) * M + (x - 1);
+  const uint32_t M = (uint32_t)T.M;
+  return (((uint32_t)(a - 1) * M + (uint32_t)(b - 1)) * M + (uint32_t)(y - 1)) * M + (uint32_t)(x - 1);
 }
 
 __device__ inline double I4(const Tabs &T, int i, int j, int k, int l) {  // <ij|kl>, 1-based
@@ -257,34 +263,41 @@ __device__ inline int spin_nperm(const uint64_t *det, int s, const SpinExc &e) {
 // reference with a/ac operators, in the reference's order
 template <int NI>
 __device__ double diag_H_incremental(const Tabs &T, const uint64_t *det, int na_e, int nb_e) {
+  // every array below is indexed with constants only (the spin loop and the
+  // word loops are unrolled), so it stays in registers
   const int M = T.M;
   uint64_t ref[2 * NI];
 #pragma unroll
-  for (int w = 0; w < 2 * NI; ++w) ref[w] = 0;
-  for (int o = 1; o <= na_e; ++o) ref[(o - 1) >> 6] |= 1ULL << ((o - 1) & 63);
-  for (int o = 1; o <= nb_e; ++o) ref[NI + ((o - 1) >> 6)] |= 1ULL << ((o - 1) & 63);
+  for (int w = 0; w < NI; ++w) {
+    const int na = na_e - 64 * w, nb = nb_e - 64 * w;
+    ref[w] = na <= 0 ? 0ULL : (na >= 64 ? ~0ULL : ((1ULL << na) - 1));
+    ref[NI + w] = nb <= 0 ? 0ULL : (nb >= 64 ? ~0ULL : ((1ULL << nb) - 1));
+  }
   uint64_t part[2 * NI], hole[2 * NI];
-  int nexc[2] = {0, 0};
+  int nexc0 = 0, nexc1 = 0;
 #pragma unroll
   for (int w = 0; w < 2 * NI; ++w) {
     uint64_t x = det[w] ^ ref[w];
     part[w] = x & det[w];
     hole[w] = x & ref[w];
-    nexc[w / NI] += __popcll(hole[w]);
+    if (w < NI) nexc0 += __popcll(hole[w]);
+    else nexc1 += __popcll(hole[w]);
   }
   double e = T.ref_energy;
-  if (nexc[0] + nexc[1] == 0) return e;
+  if (nexc0 + nexc1 == 0) return e;
   uint64_t key[2 * NI];
 #pragma unroll
   for (int w = 0; w < 2 * NI; ++w) key[w] = ref[w];
-  int elec[2] = {na_e, nb_e};
+#pragma unroll
   for (int s = 0; s < 2; ++s) {
-    int na = elec[s], nb = elec[1 - s];
+    int na = s == 0 ? na_e : nb_e;
+    const int nb = s == 0 ? nb_e : na_e;
+    const int nexc = s == 0 ? nexc0 : nexc1;
     // i-th particle / hole of this spin, ascending
     uint64_t pw[NI], hw[NI];
 #pragma unroll
     for (int w = 0; w < NI; ++w) { pw[w] = part[s * NI + w]; hw[w] = hole[s * NI + w]; }
-    for (int i = 0; i < nexc[s]; ++i) {
+    for (int i = 0; i < nexc; ++i) {
       int po = 0, ho = 0;
 #pragma unroll
       for (int w = 0; w < NI; ++w)
@@ -296,7 +309,8 @@ __device__ double diag_H_incremental(const Tabs &T, const uint64_t *det, int na_
       {
         e += T.hcore[(po - 1) * (M + 1)];
         int cnt = 0;
-        for (int w = 0; w < NI && cnt < na; ++w) {
+#pragma unroll
+        for (int w = 0; w < NI; ++w) {
           uint64_t y = key[s * NI + w];
           while (y && cnt < na) {
             int o = 64 * w + __ffsll((unsigned long long)y);
@@ -306,7 +320,8 @@ __device__ double diag_H_incremental(const Tabs &T, const uint64_t *det, int na_
           }
         }
         cnt = 0;
-        for (int w = 0; w < NI && cnt < nb; ++w) {
+#pragma unroll
+        for (int w = 0; w < NI; ++w) {
           uint64_t y = key[(1 - s) * NI + w];
           while (y && cnt < nb) {
             int o = 64 * w + __ffsll((unsigned long long)y);
@@ -315,16 +330,17 @@ __device__ double diag_H_incremental(const Tabs &T, const uint64_t *det, int na_
             ++cnt;
           }
         }
-        key[s * NI + ((po - 1) >> 6)] |= 1ULL << ((po - 1) & 63);
+        set_orb<NI>(key, s + 1, po);
         na += 1;
       }
       // a_operator(ho): removes ho, then loops over the remaining occupations
       {
-        key[s * NI + ((ho - 1) >> 6)] &= ~(1ULL << ((ho - 1) & 63));
+        clear_orb<NI>(key, s + 1, ho);
         na -= 1;
         e -= T.hcore[(ho - 1) * (M + 1)];
         int cnt = 0;
-        for (int w = 0; w < NI && cnt < na; ++w) {
+#pragma unroll
+        for (int w = 0; w < NI; ++w) {
           uint64_t y = key[s * NI + w];
           while (y && cnt < na) {
             int o = 64 * w + __ffsll((unsigned long long)y);
@@ -334,7 +350,8 @@ __device__ double diag_H_incremental(const Tabs &T, const uint64_t *det, int na_
           }
         }
         cnt = 0;
-        for (int w = 0; w < NI && cnt < nb; ++w) {
+#pragma unroll
+        for (int w = 0; w < NI; ++w) {
           uint64_t y = key[(1 - s) * NI + w];
           while (y && cnt < nb) {
             int o = 64 * w + __ffsll((unsigned long long)y);
@@ -442,7 +459,59 @@ struct NearArgs {
   int *cnt;            // counts (per generator)
   int *list;           // output det indices (1-based)
   int *past;           // per generator: kept entries with index < the generator
+  uint64_t *sig;       // output hole signatures (hole_sig), parallel to list
 };
+
+// The holes of d relative to the generator (orbitals the generator occupies
+// and d does not; at most 4 for a near-list entry), spin 1 first, ascending:
+// 16-bit fields (s - 1) << 8 | (o - 1), unused fields 0xffff.  A block
+// (h1, h2) then tests an entry without its determinant: the entry has
+// e + [h1 not a hole] + [h2 not a hole] particles outside the block's mask,
+// e its number of holes.
+template <int NI>
+__device__ inline uint64_t hole_sig(const uint64_t *gen, const uint64_t *d) {
+  uint64_t sig = ~0ULL;
+  int n = 0;
+#pragma unroll
+  for (int s = 1; s <= 2; ++s)
+#pragma unroll
+    for (int w = 0; w < NI; ++w) {
+      uint64_t hh = spin_word<NI>(gen, s, w) & ~spin_word<NI>(d, s, w);
+      while (hh) {
+        const int o = 64 * w + __ffsll((unsigned long long)hh) - 1;
+        hh &= hh - 1;
+        if (n < 4) sig = (sig & ~(0xffffULL << (16 * n))) | ((uint64_t)(((s - 1) << 8) | o) << (16 * n));
+        ++n;
+      }
+    }
+  return sig;
+}
+__device__ inline uint32_t hole_code(int s, int o) { return (uint32_t)(((s - 1) << 8) | (o - 1)); }
+// particles outside the mask of block (c1, c2) of an entry with signature sig
+__device__ inline int sig_nt(uint64_t sig, uint32_t c1, uint32_t c2) {
+  int e = 0;
+  bool m1 = false, m2 = false;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const uint32_t f = (uint32_t)(sig >> (16 * t)) & 0xffffu;
+    if (f != 0xffffu) {
+      ++e;
+      m1 = m1 || f == c1;
+      m2 = m2 || f == c2;
+    }
+  }
+  return e + (m1 ? 0 : 1) + (m2 ? 0 : 1);
+}
+// every hole of the entry is h1 or h2: it contains the block's mask
+__device__ inline bool sig_in_mask(uint64_t sig, uint32_t c1, uint32_t c2) {
+  bool ok = true;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const uint32_t f = (uint32_t)(sig >> (16 * t)) & 0xffffu;
+    ok = ok && (f == 0xffffu || f == c1 || f == c2);
+  }
+  return ok;
+}
 
 template <int NI>
 __global__ __launch_bounds__(256) void near_kernel(NearArgs A, int fill) {
@@ -483,7 +552,10 @@ __global__ __launch_bounds__(256) void near_kernel(NearArgs A, int fill) {
       if (x < wv) pre += wcount[x];
       tot += wcount[x];
     }
-    if (fill && keep) A.list[out0 + base + pre + my] = (int)(i + 1);
+    if (fill && keep) {
+      A.list[out0 + base + pre + my] = (int)(i + 1);
+      A.sig[out0 + base + pre + my] = hole_sig<NI>(gen, A.psi + i * 2 * NI);
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
       base += tot;
@@ -534,6 +606,7 @@ struct SelArgs {
   double *mat_g;       // coupling matrices in HBM (one NS*M*M slot per workgroup), when they do not fit LDS
   const double *fockg; // fock_diag_tmp of each generator of the batch, 2*(M+1) per near-list slot
   const int *near_past;   // per near-list slot: entries with index < the generator (a prefix of the list)
+  const uint64_t *near_sig;   // hole signature of each near-list entry
 };
 
 // fock_diag_tmp (Fock_diag.irp.f:1-93) of each generator of a batch: the
@@ -733,15 +806,15 @@ struct CellSet {
     fl = 0u;
   }
   __device__ inline bool used(int k) const { return (fl >> (16 + k)) & 1u; }
-  __device__ inline void put2(int k, int i, int j, int64_t a, int64_t b, uint32_t odd) {
+  __device__ inline void put2(int k, int i, int j, uint32_t a, uint32_t b, uint32_t odd) {
     cl[k] = (uint32_t)(i - 1) | ((uint32_t)(j - 1) << 8);
-    ia[k] = (uint32_t)a;
-    ib[k] = (uint32_t)b;
+    ia[k] = a;
+    ib[k] = b;
     fl |= (1u << k) | (odd << (8 + k)) | (1u << (16 + k));
   }
-  __device__ inline void put1(int k, int i, int j, int64_t a, uint32_t odd) {
+  __device__ inline void put1(int k, int i, int j, uint32_t a, uint32_t odd) {
     cl[k] = (uint32_t)(i - 1) | ((uint32_t)(j - 1) << 8);
-    ia[k] = (uint32_t)a;
+    ia[k] = a;
     fl |= (odd << (8 + k)) | (1u << (16 + k));
   }
   // loads the integrals of the used cells and forms their values
@@ -827,8 +900,8 @@ __device__ __forceinline__ void get_d2(Ctx<NI> &C, const uint64_t *sel, const Se
   const int M = C.M;
   CellSet cs;
   cs.clear();
-  auto idx4 = [&](int i, int j, int k, int l) -> int64_t {   // <ij|kl>, 1-based
-    return pair_index(i - 1, k - 1) * (int64_t)C.T.npair + pair_index(j - 1, l - 1);
+  auto idx4 = [&](int i, int j, int k, int l) -> uint32_t {   // <ij|kl>, 1-based
+    return pair32(i - 1, k - 1) * (uint32_t)C.T.npair + pair32(j - 1, l - 1);
   };
   if (sp == 3) {
     const uint32_t *BB = (ma == 2) ? C.L.ban2 : C.L.ban1;
@@ -932,7 +1005,7 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const Se
   cs.clear();
   // <ij|kl> (1-based) with argument `vary` the orbital that changes along
   // the line (the cells' owners are consecutive threads)
-  auto idx4 = [&](int i, int j, int k, int l, int vary) -> int64_t { return line_off(C.T, i, j, k, l, vary); };
+  auto idx4 = [&](int i, int j, int k, int l, int vary) -> uint32_t { return line_off(C.T, i, j, k, l, vary); };
   const double *TT = C.T.G4 ? C.T.G4 : C.T.V;
   auto row_of = [&](int j) -> int { return owned_i(j, M); };   // i with owns(i, j), 0 if none
   if (sp == 3) {
@@ -1293,16 +1366,15 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
   __syncthreads();
 
   const int *near = A.near_list + A.near_off[B.gslot];
+  const uint64_t *nsig = A.near_sig + A.near_off[B.gslot];
   const int n_near = A.near_cnt[B.gslot];
+  const uint32_t c1 = hole_code(B.s1, B.h1), c2 = hole_code(B.s2, B.h2);
 
   // ---- spot_isinwf ----
   for (int x = tid; x < n_near; x += SB) {
+    if (!sig_in_mask(nsig[x], c1, c2)) continue;   // the entry does not contain the mask
     const int idx = near[x];
     const uint64_t *d = A.psi + (int64_t)(idx - 1) * 2 * NI;
-    bool cont = true;
-#pragma unroll
-    for (int w = 0; w < 2 * NI; ++w) cont = cont && ((d[w] & mask[w]) == mask[w]);
-    if (!cont) continue;
     if (idx < B.gen) {
       atomicOr(&L.flag[0], 1);
       continue;
@@ -1338,11 +1410,9 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
   for (int x = tid; x < n_past; x += SB) {
     const int idx = near[x];
     if (idx >= B.gen || idx > A.n_sel) continue;
-    const uint64_t *d = A.psi + (int64_t)(idx - 1) * 2 * NI;
-    int nt = 0;
-#pragma unroll
-    for (int w = 0; w < 2 * NI; ++w) nt += __popcll(~mask[w] & d[w]);
+    const int nt = sig_nt(nsig[x], c1, c2);
     if (nt < 3 || nt > 4) continue;
+    const uint64_t *d = A.psi + (int64_t)(idx - 1) * 2 * NI;
     const SelL SL = sel_lists<NI>(mask, d);
     if (nt == 4) {
       if (sp == 3) {
@@ -1419,12 +1489,21 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
     // | first record slot << 8.
     for (int e = tid; e < nq; e += SB) {
       const int kind = Q.kind_n[e];
-      if (kind == 2) continue;
+      const int idx = Q.idx[e];
+      for (int st = 0; st < NS; ++st) Q.coef[(size_t)NS * e + st] = A.coef[(idx - 1) + A.n_det * st];
+      C.gath += NS;
       uint64_t d[2 * NI];
-      const uint64_t *dp = kQueueHoldsDet<NI> ? reinterpret_cast<const uint64_t *>(Q.hij + 6 * e)
-                                              : A.psi + (int64_t)(Q.idx[e] - 1) * 2 * NI;
+      const uint64_t *dp = A.psi + (int64_t)(idx - 1) * 2 * NI;
 #pragma unroll
       for (int w = 0; w < 2 * NI; ++w) d[w] = dp[w];
+      if (kQueueHoldsDet<NI> && kind != 0) {
+        // the determinant of a three- or fewer-particle selector goes where
+        // a four-particle selector's couplings go, for the replay to read
+        uint64_t *qd = reinterpret_cast<uint64_t *>(Q.hij + 6 * e);
+#pragma unroll
+        for (int w = 0; w < 2 * NI; ++w) qd[w] = d[w];
+      }
+      if (kind == 2) continue;
       const SelL SL = sel_lists<NI>(mask, d);
       if (kind == 0) {
         D2List dl{Q.cellw + 3 * e, Q.hij + 6 * e, 0};
@@ -1550,17 +1629,15 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
     apply_until(nq);
   };
   int qn = 0;
-  // the near-list entry and determinant of the next chunk are loaded while
-  // the current one is processed
+  // the near-list entry and hole signature of the next chunk are loaded
+  // while the current one is processed; the filter needs no determinant
   int nx_idx = 0;
-  uint64_t nx_d[2 * NI];
+  uint64_t nx_sig = 0;
   auto fetch = [&](int x) {
     nx_idx = 0;
     if (x < n_near) {
       nx_idx = near[x];
-      const uint64_t *d = A.psi + (int64_t)(nx_idx - 1) * 2 * NI;
-#pragma unroll
-      for (int w = 0; w < 2 * NI; ++w) nx_d[w] = d[w];
+      nx_sig = nsig[x];
     }
   };
   // the selectors before the generator (the first n_past entries of the
@@ -1570,16 +1647,13 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
   for (int x0 = n_past;; x0 += SB) {
     const bool more = x0 < n_near;
     const int idx = nx_idx;
-    uint64_t d[2 * NI];
-#pragma unroll
-    for (int w = 0; w < 2 * NI; ++w) d[w] = nx_d[w];
+    const uint64_t sig = nx_sig;
     bool pass = false;
     int nt = 0, cnt = 0, pos = 0;
     if (more) {
       fetch(x0 + SB + tid);
       if (idx >= B.gen && idx <= A.n_sel) {   // idx == 0 past the end of the list
-#pragma unroll
-        for (int w = 0; w < 2 * NI; ++w) nt += __popcll(~mask[w] & d[w]);
+        nt = sig_nt(sig, c1, c2);
         pass = nt <= 4;
       }
       // order-preserving compaction: position of this survivor in the chunk
@@ -1598,20 +1672,10 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
     while (true) {
       const int space = A.qcap - qn;
       if (pass && pos >= done && pos < done + space) {
-        // the survivor fits in the queue: fill its entry (the four-particle
-        // couplings are computed straight into it)
+        // the survivor fits in the queue: its index and kind (its
+        // coefficients, determinant and couplings are filled at the flush)
         const int e = qn + pos - done;
         Q.idx[e] = idx;
-        for (int st = 0; st < NS; ++st) Q.coef[(size_t)NS * e + st] = A.coef[(idx - 1) + A.n_det * st];
-        C.gath += NS;
-        if (kQueueHoldsDet<NI>) {
-          // the determinant goes where a four-particle selector's couplings
-          // will be, for the flush to read from LDS
-          uint64_t *qd = reinterpret_cast<uint64_t *>(Q.hij + 6 * e);
-#pragma unroll
-          for (int w = 0; w < 2 * NI; ++w) qd[w] = d[w];
-        }
-        // kind 0 (four particles): couplings computed at the flush
         Q.kind_n[e] = nt == 4 ? 0 : (nt == 3 ? 1 : 2);
       }
       const int adv = min(cnt - done, space);
@@ -1895,6 +1959,7 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
   int64_t ncand_total = 0, ngath_total = 0, nproc = 0;
 
   DBuf<int> gens_d, near_cnt_d, near_list_d, near_past_d;
+  DBuf<uint64_t> near_sig_d;
   DBuf<double> fockg_d;
   DBuf<int64_t> near_off_d;
   DBuf<BlockDesc> blocks_d;
@@ -1966,7 +2031,7 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
     near_cnt_d.alloc(ng);
     near_off_d.alloc(ng);
     near_past_d.alloc(ng);
-    NearArgs NA{W.psi.p, n_det, n_sel, gens_d.p, ng, near_off_d.p, near_cnt_d.p, nullptr, near_past_d.p};
+    NearArgs NA{W.psi.p, n_det, n_sel, gens_d.p, ng, near_off_d.p, near_cnt_d.p, nullptr, near_past_d.p, nullptr};
     hipLaunchKernelGGL(near_kernel<NI>, dim3(ng), dim3(256), 0, st, NA, 0);
     QPX_HIP(hipGetLastError());
     std::vector<int> cnt(ng);
@@ -1977,7 +2042,9 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
     for (int i = 0; i < ng; ++i) { off[i] = tot; tot += cnt[i]; }
     near_off_d.upload(off, st);
     near_list_d.alloc(std::max<int64_t>(tot, 1));
+    near_sig_d.alloc(std::max<int64_t>(tot, 1));
     NA.list = near_list_d.p;
+    NA.sig = near_sig_d.p;
     hipLaunchKernelGGL(near_kernel<NI>, dim3(ng), dim3(256), 0, st, NA, 1);
     QPX_HIP(hipGetLastError());
     fockg_d.alloc((size_t)ng * 2 * (M + 1));
@@ -2015,7 +2082,7 @@ void run_selection(qpx_ctx *c, int NS, const double *coef, int n_gen, int n_sel,
       for (int s = 0; s < NS; ++s) SA.E0[s] = E0[s];
       SA.blocks = blocks_d.p; SA.n_blocks = nb;
       SA.near_off = near_off_d.p; SA.near_cnt = near_cnt_d.p; SA.near_list = near_list_d.p;
-      SA.near_past = near_past_d.p; SA.fockg = fockg_d.p;
+      SA.near_past = near_past_d.p; SA.fockg = fockg_d.p; SA.near_sig = near_sig_d.p;
       SA.pool_key = P.key.p; SA.pool_det = P.det.p; SA.pool_blk = P.blk.p; SA.pool_count = P.count.p;
       SA.pool_cap = P.cap; SA.tau = tau;
       SA.blk_pt2 = bpt2_d.p; SA.blk_status = bstat_d.p; SA.blk_ncand = bncand_d.p; SA.blk_ngath = bngath_d.p;
