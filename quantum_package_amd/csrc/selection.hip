@@ -1451,7 +1451,7 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
   // diagonal by past_d2 (never happens: past_d2 sets both orientations)
 
   // ---- splash_pq: couplings in selector order ----
-  // Phase A: SB near-list entries at a time are filtered in parallel and the
+  // Phase A: FR * SB near-list entries at a time are filtered in parallel and the
   // survivors appended, in near-list order, to an LDS queue with their
   // determinants.  Phase B (flush): the couplings of the four-particle
   // entries are computed, one entry per thread (get_d2); then the queue in
@@ -1629,54 +1629,69 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
     apply_until(nq);
   };
   int qn = 0;
-  // the near-list entry and hole signature of the next chunk are loaded
-  // while the current one is processed; the filter needs no determinant
-  int nx_idx = 0;
-  uint64_t nx_sig = 0;
-  auto fetch = [&](int x) {
-    nx_idx = 0;
-    if (x < n_near) {
-      nx_idx = near[x];
-      nx_sig = nsig[x];
-    }
-  };
+  // Each round, every thread tests FR consecutive near-list entries from
+  // their hole signatures alone (no determinant is loaded); the survivors
+  // keep their near-list order through a workgroup scan of the per-thread
+  // counts.
+  constexpr int FR = 4;
   // the selectors before the generator (the first n_past entries of the
   // near list) never pass the filter: the scan starts after them
-  fetch(n_past + tid);
   // one extra round past the end of the list flushes what is left
-  for (int x0 = n_past;; x0 += SB) {
+  for (int x0 = n_past;; x0 += FR * SB) {
     const bool more = x0 < n_near;
-    const int idx = nx_idx;
-    const uint64_t sig = nx_sig;
-    bool pass = false;
-    int nt = 0, cnt = 0, pos = 0;
+    int idx[FR];
+    uint32_t passm = 0, kinds = 0;   // bit j: entry j survives; 2 bits per entry: its kind
+    int cnt = 0, pos0 = 0;
     if (more) {
-      fetch(x0 + SB + tid);
-      if (idx >= B.gen && idx <= A.n_sel) {   // idx == 0 past the end of the list
-        nt = sig_nt(sig, c1, c2);
-        pass = nt <= 4;
+      const int xt = x0 + FR * tid;
+#pragma unroll
+      for (int j = 0; j < FR; ++j) {
+        idx[j] = 0;
+        uint64_t sg = 0;
+        if (xt + j < n_near) {
+          idx[j] = near[xt + j];
+          sg = nsig[xt + j];
+        }
+        if (idx[j] >= B.gen && idx[j] <= A.n_sel) {   // idx == 0 past the end of the list
+          const int nt = sig_nt(sg, c1, c2);
+          if (nt <= 4) {
+            passm |= 1u << j;
+            kinds |= (uint32_t)(nt == 4 ? 0 : (nt == 3 ? 1 : 2)) << (2 * j);
+          }
+        }
       }
-      // order-preserving compaction: position of this survivor in the chunk
-      const unsigned long long bal = __ballot(pass);
-      const int before = __popcll(bal & ((1ULL << lane) - 1ULL));
-      if (lane == 0) wsum[wv] = __popcll(bal);
+      // order-preserving compaction: exclusive scan of the survivor counts
+      const int c = __popc(passm);
+      int incl = c;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+      }
+      if (lane == 63) wsum[wv] = incl;
       __syncthreads();
       int off = 0;
       for (int w = 0; w < SB / 64; ++w) {
         if (w < wv) off += wsum[w];
         cnt += wsum[w];
       }
-      pos = off + before;
+      pos0 = off + incl - c;
     }
     int done = 0;
     while (true) {
       const int space = A.qcap - qn;
-      if (pass && pos >= done && pos < done + space) {
-        // the survivor fits in the queue: its index and kind (its
-        // coefficients, determinant and couplings are filled at the flush)
-        const int e = qn + pos - done;
-        Q.idx[e] = idx;
-        Q.kind_n[e] = nt == 4 ? 0 : (nt == 3 ? 1 : 2);
+      int pos = pos0;
+#pragma unroll
+      for (int j = 0; j < FR; ++j) {
+        if (!((passm >> j) & 1u)) continue;
+        if (pos >= done && pos < done + space) {
+          // the survivor fits in the queue: its index and kind (its
+          // coefficients, determinant and couplings are filled at the flush)
+          const int e = qn + pos - done;
+          Q.idx[e] = idx[j];
+          Q.kind_n[e] = (int)((kinds >> (2 * j)) & 3u);
+        }
+        ++pos;
       }
       const int adv = min(cnt - done, space);
       qn += adv;
