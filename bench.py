@@ -388,7 +388,7 @@ def selected_davidson_leg(args, log):
     import numpy as np
     import torch
     from quantum_package_amd.api import FP32, FP64, Context
-    from quantum_package_amd.cipsi import cipsi, make_s2_eigenfunction, n_leading, sort_by_weight
+    from quantum_package_amd.cipsi import cipsi, n_leading, sort_by_weight
     from quantum_package_amd.hamiltonian import synthetic_hamiltonian
     M, na, nb, seed, ns, nst = 96, 10, 10, 1003, 2, 4
     H = synthetic_hamiltonian(M, na, nb, seed=seed)
@@ -403,7 +403,10 @@ def selected_davidson_leg(args, log):
     E0 = np.array([ctx.u_0_H_u_0(cs[:, k])[0] for k in range(ns)])
     _, new, _ = ctx.ZMQ_selection(np.asfortranarray(cs), n_gen, n_sel, E0, args.sel_select)
     dets = np.concatenate([ds, new])
-    dets = np.concatenate([dets, make_s2_eigenfunction(dets, na)])
+    t_s2 = time.time()
+    extra = ctx.s2_complete(dets, na)   # make_s2_eigenfunction on the GPU
+    t_s2 = time.time() - t_s2
+    dets = np.concatenate([dets, extra])
     nd = dets.shape[0]
     t_build = time.time() - t0
     log(f"bench: selected psi {nd} dets ({n} by the CIPSI loop, {new.shape[0]} selected in one step, rest S^2 "
@@ -413,7 +416,9 @@ def selected_davidson_leg(args, log):
     visited = ctx._sparse_stats()
     t_buckets = time.time() - t0
     path = ctx._force_sigma_path(0)
-    res = {"n_det": nd, "psi_build_seconds": t_build, "bucket_build_seconds": t_buckets, "visited": visited}
+    res = {"n_det": nd, "psi_build_seconds": t_build, "s2_complete": {"seconds": t_s2, "input": int(nd - extra.shape[0]),
+                                                                       "added": int(extra.shape[0])},
+           "bucket_build_seconds": t_buckets, "visited": visited}
     # sigma kernel timings, 4 vectors, FP64 and FP32; measured HBM bytes per
     # call (rocprofv3 FETCH_SIZE/WRITE_SIZE of this leg, calibrated) when the
     # committed summary is for this psi
@@ -502,7 +507,7 @@ def cipsi_step_leg(args, log):
     """Plain vs learned CIPSI step on a CIPSI-grown psi."""
     import numpy as np
     from quantum_package_amd.api import BlockMLP, Context
-    from quantum_package_amd.cipsi import cipsi, make_s2_eigenfunction, n_leading, sort_by_weight
+    from quantum_package_amd.cipsi import cipsi, n_leading, sort_by_weight
     from quantum_package_amd.hamiltonian import synthetic_hamiltonian
     from quantum_package_amd.proposal import Proposer, block_descriptors
     M, na, nb, seed, ns = 96, 10, 10, 1003, 2
@@ -554,7 +559,7 @@ def cipsi_step_leg(args, log):
         f"bound at 5e-3 missed: {bound['5.00e-03']['selection_speedup']:.2f}x")
     # Davidson on the grown psi (the full sweep's determinants), FP64 then mixed
     dets = np.concatenate([ds, new_full])
-    extra = make_s2_eigenfunction(dets, na)
+    extra = ctx.s2_complete(dets, na)
     dets = np.concatenate([dets, extra])
     g = np.zeros((dets.shape[0], ns))
     g[:n] = cs

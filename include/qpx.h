@@ -156,6 +156,16 @@ int qpx_select_tasks(qpx_ctx *ctx, int n_states, const double *coef, int n_sel, 
 int qpx_make_s2_eigenfunction(int n_int, int64_t n_det, const uint64_t *psi_det, int n_alpha, int64_t max_out,
                               uint64_t *out_dets, int64_t *n_out);
 
+/* make_s2_eigenfunction (src/Determinants/occ_pattern.irp.f:251-309) on the
+ * context's GPU: the same determinants, in the same order, as
+ * qpx_make_s2_eigenfunction for psi_det (N_int,2,n_det) (host arrays in and
+ * out).  *n_out receives their count; with out_dets null only the count is
+ * computed; with max_out < *n_out nothing is written and 4 is returned.
+ * Patterns with more than 64 open shells, or groups whose alpha subsets
+ * number above 2^40, are rejected. */
+int qpx_s2_complete(qpx_ctx *ctx, int n_int, int64_t n_det, const uint64_t *psi_det, int n_alpha, int64_t max_out,
+                    uint64_t *out_dets, int64_t *n_out);
+
 #ifdef __cplusplus
 }
 #endif

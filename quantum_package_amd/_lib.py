@@ -18,6 +18,7 @@ c_vp = C.c_void_p
 SIGNATURES = [
     ("qpx_create", C.c_int, [C.c_int, C.POINTER(c_vp)]),
     ("qpx_make_s2_eigenfunction", C.c_int, [C.c_int, C.c_int64, c_u64p, C.c_int, C.c_int64, c_u64p, c_i64p]),
+    ("qpx_s2_complete", C.c_int, [c_vp, C.c_int, C.c_int64, c_u64p, C.c_int, C.c_int64, c_u64p, c_i64p]),
     ("qpx_select_generators", C.c_int, [c_vp, C.c_int, c_dp, C.c_int, c_dp, C.c_int, c_ip, c_dp]),
     ("qpx_select_tasks", C.c_int, [c_vp, C.c_int, c_dp, C.c_int, c_dp, C.c_int64, C.c_int, c_ip, c_ip, c_dp,
                                    c_u64p, c_dp, c_i64p, c_dp]),

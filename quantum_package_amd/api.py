@@ -167,6 +167,23 @@ class Context:
                                                  g.ctypes.data_as(_lib.c_ip), _dp(out)))
         return out
 
+    def s2_complete(self, dets, n_alpha: int):
+        """make_s2_eigenfunction on the GPU (occ_pattern.irp.f:251-309): the
+        determinants sharing an occupation pattern with dets (N, 2, n_int)
+        and not in it, in the order of the host routine
+        (cipsi.make_s2_eigenfunction)."""
+        d = np.ascontiguousarray(dets, dtype=np.uint64)
+        n_int = d.shape[2]
+        n = C.c_int64(0)
+        self._check(self.L.qpx_s2_complete(self.ptr, n_int, d.shape[0], d.ctypes.data_as(_lib.c_u64p), int(n_alpha),
+                                           0, None, C.byref(n)))
+        out = np.empty((n.value, 2, n_int), dtype=np.uint64)
+        if n.value:
+            self._check(self.L.qpx_s2_complete(self.ptr, n_int, d.shape[0], d.ctypes.data_as(_lib.c_u64p),
+                                               int(n_alpha), n.value, out.ctypes.data_as(_lib.c_u64p),
+                                               C.byref(n)))
+        return out
+
     def select_tasks(self, coef_sorted, n_sel: int, E0, N: int, tasks):
         """The selection task farm over explicit (i_generator, subset) tasks
         (1-based generator indices into the current psi_det_sorted; subset 0

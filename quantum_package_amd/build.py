@@ -25,7 +25,7 @@ FLAGS = [
 ]
 
 SOURCES = ["context.hip", "wavefunction.hip", "sigma.hip", "sigma_sparse.hip", "davidson.hip", "selection.hip", "mlp.hip",
-           "psi_update.hip"]
+           "psi_update.hip", "s2_complete.hip"]
 
 
 # the selection kernel keeps the reference's floating-point operation order

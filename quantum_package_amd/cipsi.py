@@ -16,7 +16,8 @@ Per iteration:
   4. fill_H_apply_buffer_no_selection + copy_H_apply_buffer_to_wf: the new
      determinants enter psi with zero coefficients and psi is normalised
      (src/Determinants/H_apply.irp.f:113-198,275-316);
-  5. make_s2_eigenfunction when s2_eig or N_states > 1: every determinant
+  5. make_s2_eigenfunction (on the GPU, Context.s2_complete) when s2_eig or
+     N_states > 1: every determinant
      sharing an occupation pattern with psi is added
      (src/Determinants/occ_pattern.irp.f:251-309).
 
@@ -201,7 +202,7 @@ def cipsi(ctx, H, n_det_max: int, n_states: int = 1, n_states_diag: int = 4, s2_
         dets = np.concatenate([dets_s, new_dets], axis=0)
         coef = np.concatenate([coef_s, np.zeros((new_dets.shape[0], n_states))], axis=0)
         if s2_eig or n_states > 1:
-            extra = make_s2_eigenfunction(dets, na)
+            extra = ctx.s2_complete(dets, na)
             if extra.shape[0]:
                 dets = np.concatenate([dets, extra], axis=0)
                 coef = np.concatenate([coef, np.zeros((extra.shape[0], n_states))], axis=0)

@@ -94,6 +94,15 @@ module qpx_mod
       integer(c_int64_t) :: psi(*), dets(*)
       integer(c_int64_t) :: nout
     end function
+    integer(c_int) function qpx_s2_complete(ctx, nint, ndet, psi, nalpha, maxout, dets, nout) &
+        bind(C, name="qpx_s2_complete")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nint, nalpha
+      integer(c_int64_t), value :: ndet, maxout
+      integer(c_int64_t) :: psi(*), dets(*)
+      integer(c_int64_t) :: nout
+    end function
     integer(c_int) function qpx_select(ctx, nstates, coef, ngen, nsel, e0, nkeep, subset, pt2, dets, vals, nout) &
         bind(C, name="qpx_select")
       import :: c_int, c_ptr, c_double, c_int64_t
@@ -226,6 +235,25 @@ contains
     rc = qpx_make_s2_eigenfunction(int(nint, c_int), int(ndet, c_int64_t), psi_det, int(n_alpha, c_int), &
         nout, new_dets, nout)
     if (rc /= 0) stop 'qpx_make_s2_eigenfunction failed'
+    n_new = nout
+  end subroutine
+
+  ! the same on the GPU of the module's context (qpx_s2_complete), same
+  ! determinants in the same order
+  subroutine qpx_s2_completion_gpu(psi_det, nint, ndet, n_alpha, new_dets, n_new)
+    integer, intent(in) :: nint, ndet, n_alpha
+    integer(8), intent(in) :: psi_det(nint, 2, ndet)
+    integer(8), allocatable, intent(out) :: new_dets(:,:,:)
+    integer(8), intent(out) :: n_new
+    integer(c_int64_t) :: nout, dummy(1)
+    integer(c_int) :: rc
+    rc = qpx_s2_complete(qpx_ctx, int(nint, c_int), int(ndet, c_int64_t), psi_det, int(n_alpha, c_int), &
+        0_c_int64_t, dummy, nout)
+    if (rc /= 0 .and. rc /= 4) stop 'qpx_s2_complete failed'
+    allocate(new_dets(nint, 2, max(nout, 1_c_int64_t)))
+    rc = qpx_s2_complete(qpx_ctx, int(nint, c_int), int(ndet, c_int64_t), psi_det, int(n_alpha, c_int), &
+        nout, new_dets, nout)
+    if (rc /= 0) stop 'qpx_s2_complete failed'
     n_new = nout
   end subroutine
 

@@ -8,7 +8,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from quantum_package_amd.cipsi import cipsi
+from quantum_package_amd.cipsi import cipsi, make_s2_eigenfunction
 from quantum_package_amd.hamiltonian import synthetic_hamiltonian
 from oracle import oracle as O
 from test_gpu_sigma import _say
@@ -44,6 +44,10 @@ class OracleBackend:
     def ZMQ_selection(self, coef, n_gen, n_sel, E0, N):
         r = O.selection(self.ham, self.dets, coef, n_gen, n_sel, E0, N)
         return r["pt2"], r["dets"], r["vals"]
+
+    def s2_complete(self, dets, n_alpha):
+        # the host routine, pinned by brute force in test_cpu_cipsi.py
+        return make_s2_eigenfunction(dets, n_alpha)
 
 
 @pytest.mark.parametrize("s2_eig", [False, True])
