@@ -308,7 +308,9 @@ __global__ __launch_bounds__(1024) void sigma_dense_row_kernel(DenseArgs A, int 
   const int per_xcd = gridDim.x >> 3;
   const int logical = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
   if (logical >= A.n_a * nchunk) return;
-  const int a = logical / nchunk, chunk = logical - a * nchunk;
+  // chunk-major: the workgroups an XCD runs together share a chunk of beta
+  // strings, whose same-spin connection lists then stay in its L2
+  const int chunk = logical / A.n_a, a = logical - chunk * A.n_a;
   const int r0 = A.row_ptr[a], r1 = A.row_ptr[a + 1];
   const int n_row = r1 - r0;
   const int tid = threadIdx.x, nth = blockDim.x;
