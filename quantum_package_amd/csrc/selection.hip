@@ -71,7 +71,11 @@ __device__ inline bool bit(const uint64_t *s, int o) {  // o 1-based
 // These helpers pick the word with selects instead.
 template <int NI>
 __device__ __forceinline__ uint64_t spin_word(const uint64_t *a, int s, int w) {   // w a constant
-  return s == 1 ? a[w] : a[NI + w];
+  uint64_t x0 = a[w], x1 = a[NI + w];
+  // the empty asm keeps the selection on the two values: folded into one
+  // load from a selected address, it would be a run-time index again
+  asm("" : "+v"(x0), "+v"(x1));
+  return s == 1 ? x0 : x1;
 }
 // the word of spin s (1-based) holding orbital o (1-based)
 template <int NI>
@@ -88,16 +92,14 @@ __device__ __forceinline__ void set_orb(uint64_t *a, int s, int o) {
   const int idx = (s - 1) * NI + ((o - 1) >> 6);
   const uint64_t b = 1ULL << ((o - 1) & 63);
 #pragma unroll
-  for (int w = 0; w < 2 * NI; ++w)
-    if (w == idx) a[w] |= b;
+  for (int w = 0; w < 2 * NI; ++w) a[w] |= b & (0ULL - (uint64_t)(w == idx));   // branch-free: no merged store
 }
 template <int NI>
 __device__ __forceinline__ void clear_orb(uint64_t *a, int s, int o) {
   const int idx = (s - 1) * NI + ((o - 1) >> 6);
   const uint64_t b = 1ULL << ((o - 1) & 63);
 #pragma unroll
-  for (int w = 0; w < 2 * NI; ++w)
-    if (w == idx) a[w] &= ~b;
+  for (int w = 0; w < 2 * NI; ++w) a[w] &= ~(b & (0ULL - (uint64_t)(w == idx)));
 }
 
 // parity of the occupied orbitals with index <= o in spin s of det
