@@ -1010,22 +1010,31 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const Se
   auto idx4 = [&](int i, int j, int k, int l, int vary) -> uint32_t { return line_off(C.T, i, j, k, l, vary); };
   const double *TT = C.T.G4 ? C.T.G4 : C.T.V;
   auto row_of = [&](int j) -> int { return owned_i(j, M); };   // i with owns(i, j), 0 if none
+  // phases: odd_pm's terms of the orbitals fixed for the whole selector are
+  // computed once; a cell adds those of its varying orbital
+  auto pb = [&](int s, int o) -> uint32_t { return (uint32_t)pm_bit<NI>(pmk, s, o); };
+  // same-spin double (h, hfix) -> (p1, p2) with h the cell's varying orbital
+  auto same_odd = [&](uint32_t fixed, int h, int hfix, int p1, int p2) -> uint32_t {
+    const int h1 = min(h, hfix), h2 = max(h, hfix);
+    return fixed ^ pb(ma, h) ^ (uint32_t)(p1 < h1) ^ (uint32_t)(p2 < h2) ^ (uint32_t)(max(h1, p1) > min(h2, p2));
+  };
   if (sp == 3) {
     const uint32_t *BB = (ma == 2) ? C.L.ban2 : C.L.ban1;
     const int puti = P(1, mi), hfix = H(1, ma), p1 = P(1, ma), p2 = P(2, ma);
+    const uint32_t f_ma = pb(ma, hfix) ^ pb(ma, p1) ^ pb(ma, p2);
     if (!borb_get(BO, puti, mi)) {
       const int putj = (ma == 1) ? row_of(puti) : owned_j(puti, M);
       if (putj && putj != hfix && !lbanned(putj, ma) && !ban_get(BB, M, putj, puti)) {
         const int ci = (ma == 1) ? putj : puti, cj = (ma == 1) ? puti : putj;
+        const uint32_t odd = same_odd(f_ma, putj, hfix, p1, p2);
         if (putj < hfix)
-          cs.put2(0, ci, cj, idx4(p1, p2, putj, hfix, 2), idx4(p2, p1, putj, hfix, 2),
-                  odd_pm<NI>(pmk, ma, ma, putj, p1, hfix, p2));
+          cs.put2(0, ci, cj, idx4(p1, p2, putj, hfix, 2), idx4(p2, p1, putj, hfix, 2), odd);
         else
-          cs.put2(0, ci, cj, idx4(p1, p2, hfix, putj, 3), idx4(p2, p1, hfix, putj, 3),
-                  odd_pm<NI>(pmk, ma, ma, hfix, p1, putj, p2));
+          cs.put2(0, ci, cj, idx4(p1, p2, hfix, putj, 3), idx4(p2, p1, hfix, putj, 3), odd);
       }
     }
     const int pfix = P(1, mi);
+    const uint32_t f_mi = pb(ma, hfix) ^ pb(mi, pfix);
     // lines with fixed ma-orbital p1 and p2, varying mi-orbital
 #pragma unroll
     for (int which = 0; which < 2; ++which) {
@@ -1034,16 +1043,19 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const Se
       const int pu = (mi == 1) ? row_of(pf) : owned_j(pf, M);
       if (!pu || lbanned(pu, mi)) continue;
       if (ban_get(BB, M, pf, pu)) continue;
-      cs.put1(1 + which, (mi == 1) ? pu : pf, (mi == 1) ? pf : pu, idx4(other, pfix, hfix, pu, 3),
-              odd_pm<NI>(pmk, ma, mi, hfix, other, pu, pfix));
+      // odd_pm(ma, mi, hfix, other, pu, pfix), two spins
+      const uint32_t odd = f_mi ^ pb(ma, other) ^ (uint32_t)(other < hfix) ^ pb(mi, pu) ^ (uint32_t)(pfix < pu);
+      cs.put1(1 + which, (mi == 1) ? pu : pf, (mi == 1) ? pf : pu, idx4(other, pfix, hfix, pu, 3), odd);
     }
   } else if (SL.np(ma) == 3) {
     const int hfix = H(1, ma);
+    const uint32_t f_all = pb(ma, hfix) ^ pb(ma, P(1, ma)) ^ pb(ma, P(2, ma)) ^ pb(ma, P(3, ma));
 #pragma unroll
     for (int i = 1; i <= 3; ++i) {
       const int puti = P(i, ma);
       const int i1 = (i == 1) ? 2 : 1, i2 = (i == 3) ? 2 : 3;
       const int p1 = P(i1, ma), p2 = P(i2, ma);
+      const uint32_t f_i = f_all ^ pb(ma, puti);   // hfix, p1 and p2
       // cells {x, puti}: (x, puti) for x < puti, (puti, x) for x >= puti
 #pragma unroll
       for (int half = 0; half < 2; ++half) {
@@ -1060,20 +1072,21 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const Se
         if (putj == hfix) continue;
         if (lbanned(putj, ma) || ban_get(B1, M, puti, putj)) continue;
         const int ci = half == 0 ? x : puti, cj = half == 0 ? puti : x;
+        const uint32_t odd = same_odd(f_i, putj, hfix, p1, p2);
         if (putj < hfix)
-          cs.put2(k, ci, cj, idx4(p1, p2, putj, hfix, 2), idx4(p2, p1, putj, hfix, 2),
-                  odd_pm<NI>(pmk, ma, ma, putj, p1, hfix, p2));
+          cs.put2(k, ci, cj, idx4(p1, p2, putj, hfix, 2), idx4(p2, p1, putj, hfix, 2), odd);
         else
-          cs.put2(k, ci, cj, idx4(p1, p2, hfix, putj, 3), idx4(p2, p1, hfix, putj, 3),
-                  odd_pm<NI>(pmk, ma, ma, hfix, p1, putj, p2));
+          cs.put2(k, ci, cj, idx4(p1, p2, hfix, putj, 3), idx4(p2, p1, hfix, putj, 3), odd);
       }
     }
   } else {
     const int hfix = H(1, mi), pfix = P(1, mi), p1 = P(1, ma), p2 = P(2, ma);
+    const uint32_t f_mi = pb(mi, hfix) ^ pb(mi, pfix) ^ (uint32_t)(pfix < hfix);
 #pragma unroll
     for (int which = 0; which < 2; ++which) {
       const int pc = which == 0 ? p2 : p1;   // line {x, pc}
       const int other = which == 0 ? p1 : p2;
+      const uint32_t f_w = f_mi ^ pb(ma, other);
 #pragma unroll
       for (int half = 0; half < 2; ++half) {
         const int k = 2 * which + half;
@@ -1088,8 +1101,9 @@ __device__ __forceinline__ void get_d1(Ctx<NI> &C, const uint64_t *sel, const Se
         const int puti = x;
         if (lbanned(puti, ma)) continue;
         if (ban_get(B1, M, puti, pc)) continue;
+        // odd_pm(mi, ma, hfix, pfix, puti, other), two spins
         cs.put1(k, half == 0 ? x : pc, half == 0 ? pc : x, idx4(pfix, other, hfix, puti, 3),
-                odd_pm<NI>(pmk, mi, ma, hfix, pfix, puti, other));
+                f_w ^ pb(ma, puti) ^ (uint32_t)(other < puti));
       }
     }
   }
