@@ -53,6 +53,7 @@ def ctx():
     (70, 3, 3, 300, 2, 3),      # two words
     (96, 10, 10, 20000, 2, 4),  # the 96-orbital system's size class
     (130, 6, 5, 3000, 2, 5),    # three words
+    (200, 5, 4, 3000, 2, 6),    # four words
 ])
 def test_s2_complete_matches_host(ctx, M, na, nb, n, max_exc, seed):
     _say(f"[s2] M={M} {na}a{nb}b n={n} seed={seed}")

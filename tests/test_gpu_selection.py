@@ -47,6 +47,11 @@ CASES = [
     ("M12-4a3b-gen-subset", 12, 4, 3, 150, 23, 1, 40, 100),
     ("M70-nint2", 70, 3, 3, 120, 24, 1, None, None),
     ("M16-5a5b", 16, 5, 5, 400, 25, 1, 200, 400),
+    # three and four words per spin string; 130 orbitals keep the coupling
+    # matrix in LDS for one state and in HBM for two; 200 orbitals in HBM
+    ("M130-nint3", 130, 3, 3, 80, 26, 1, None, None),
+    ("M130-nint3-2states-subset", 130, 4, 3, 150, 27, 2, 60, 120),
+    ("M200-nint4", 200, 3, 2, 60, 28, 1, None, None),
 ]
 
 
