@@ -60,6 +60,10 @@ CASES = [
     ("nint2-M70", 70, 3, 3, 2500, 8),
     ("nint2-M96", 96, 5, 5, 2000, 9),
     ("fci-M12-3a1b", 12, 3, 1, None, 10),
+    # three and four words per spin string (the dense map does not apply;
+    # above 160 orbitals the sparse walk reads the packed integrals, no M^4 table)
+    ("nint3-M130", 130, 4, 3, 1500, 11),
+    ("nint4-M200", 200, 3, 3, 1000, 12),
 ]
 
 
