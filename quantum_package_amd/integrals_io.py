@@ -1,4 +1,5 @@
-"""MO integrals as the text triplet ``kinetic_mo``, ``nuclear_mo``, ``bielec_mo``.
+"""MO integrals as the text triplet ``kinetic_mo``, ``nuclear_mo``, ``bielec_mo``,
+and the two-electron integral work file ``work/mo_ints`` (at the end).
 
 The reference's ``read_integrals`` program
 (``plugins/read_integral/read_integrals_mo.irp.f:1-116``) is its route for
@@ -140,3 +141,77 @@ def write_integrals_mo(H: Hamiltonian, directory: str, zero_threshold: float = 0
     with open(os.path.join(directory, "bielec_mo"), "w") as f:
         for (i, j, k, l), v in zip(idx[keep], H.eri[keep]):
             f.write(f"{i} {j} {k} {l} {v:.17g}\n")
+
+
+# ---------------------------------------------------------------------------
+# The integral work file: the consolidated map that the reference writes to
+# <ezfio>/work/mo_ints (src/Integrals_Bielec/mo_bi_integrals.irp.f:35,149;
+# src/Utils/map_functions.irp.f:1-131).  A key (bielec_integrals_index) goes
+# to bucket key >> 15 (map_shift) and is stored there as its low 15 bits, an
+# integer*2 (map_module.f90:26-27, map_append :489-517); map_size = keymax >>
+# 15 with keymax the key of <MM|MM> (map_integrals.irp.f:286-294).  Three
+# files of raw native-endian arrays:
+#   <base>_consolidated_idx    int64 (map_size + 2): 1-based start of each
+#                              bucket's entries, then n_elements + 1;
+#   <base>_consolidated_key    int16 (n_elements): the low 15 bits, ascending
+#                              within a bucket (map_sort before the save);
+#   <base>_consolidated_value  float64 (n_elements).
+# Keys absent from the map are zero integrals.
+
+MAP_SHIFT = 15
+
+
+def _map_paths(base: str) -> tuple[str, str, str]:
+    return base + "_consolidated_idx", base + "_consolidated_key", base + "_consolidated_value"
+
+
+def write_mo_ints(eri: np.ndarray, base: str, zero_threshold: float = 0.0) -> int:
+    """Write packed MO integrals (element key-1 holds <ij|kl>) as the
+    consolidated map files at base (e.g. <ezfio>/work/mo_ints).  Entries with
+    |value| <= zero_threshold are left out, as the reference keeps only the
+    integrals above its threshold.  Returns the number of stored entries."""
+    eri = np.ascontiguousarray(eri, dtype=np.float64)
+    n8 = eri.size
+    map_size = n8 >> MAP_SHIFT
+    keys = np.nonzero(np.abs(eri) > zero_threshold)[0].astype(np.int64) + 1
+    counts = np.bincount(keys >> MAP_SHIFT, minlength=map_size + 1)
+    idx = np.empty(map_size + 2, dtype="<i8")
+    idx[0] = 1
+    idx[1:] = 1 + np.cumsum(counts)
+    d = os.path.dirname(base)
+    if d:
+        os.makedirs(d, exist_ok=True)
+    p_idx, p_key, p_val = _map_paths(base)
+    idx.tofile(p_idx)
+    (keys & ((1 << MAP_SHIFT) - 1)).astype("<i2").tofile(p_key)
+    eri[keys - 1].astype("<f8").tofile(p_val)
+    return int(keys.size)
+
+
+def read_mo_ints(base: str, mo_num: int) -> np.ndarray:
+    """Packed MO integrals (n_packed(mo_num) doubles, element key-1) from the
+    consolidated map files at base, with map_load_from_disk's checks: the
+    index file must hold map_size + 2 entries, a cache key must not be
+    negative and a value must not exceed 1e30 ('Error in integrals file')."""
+    n8 = n_packed(mo_num)
+    map_size = n8 >> MAP_SHIFT
+    p_idx, p_key, p_val = _map_paths(base)
+    idx = np.fromfile(p_idx, dtype="<i8")
+    if idx.size != map_size + 2:
+        raise ValueError(f"{p_idx}: {idx.size} entries, {map_size + 2} expected for mo_num = {mo_num}")
+    n_el = int(idx[-1] - 1)
+    if idx[0] != 1 or n_el < 0 or np.any(np.diff(idx) < 0):
+        raise ValueError(f"{p_idx}: bucket starts are not 1-based and non-decreasing")
+    ckey = np.fromfile(p_key, dtype="<i2", count=n_el)
+    vals = np.fromfile(p_val, dtype="<f8", count=n_el)
+    if ckey.size != n_el or vals.size != n_el:
+        raise ValueError(f"{base}: fewer than the {n_el} entries its index announces")
+    if np.any(ckey < 0) or np.any(vals > 1.0e30):
+        raise ValueError(f"{base}: error in integrals file")
+    bucket = np.repeat(np.arange(map_size + 1, dtype=np.int64), np.diff(idx))
+    keys = (bucket << MAP_SHIFT) | ckey.astype(np.int64)
+    if np.any(keys < 1) or np.any(keys > n8):
+        raise ValueError(f"{base}: a key lies outside 1..{n8}")
+    eri = np.zeros(n8)
+    eri[keys - 1] = vals
+    return eri
