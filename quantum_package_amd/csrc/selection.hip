@@ -1286,12 +1286,15 @@ __device__ inline double diag_fock(const Tabs &T, const double *f, int sp, int h
 }
 
 // The LDS queue of a block's surviving selectors, as arrays (qcap entries):
-// per entry its index, kind (0: four particles outside the mask, couplings
-// precomputed; 1: three; 2: two or fewer) with the number of precomputed
-// couplings, the couplings' cells ((i - 1) | (j - 1) << 8, two per word) and values, and
-// its coefficients.  The replay of a flush keeps, per owner thread, the list
-// of (entry, coupling) records of the four-particle entries that fall on
-// its cells (rec, cursor), and the positions of the other entries (k12).
+// per entry its index, kind (0: four particles outside the mask; 1: three;
+// 2: two or fewer) with the number and first slot of its records, six
+// value slots (hij) with their cells ((i - 1) | (j - 1) << 8, two per word)
+// and its coefficients.  A four-particle entry's slots hold its couplings;
+// a three-particle entry's hold its determinant (N_int <= 3) and then its
+// folded single excitations; another entry's hold its determinant.  The
+// replay of a flush keeps, per owner thread, the list of (entry, slot)
+// records that fall on its cells (rec, cursor), and the positions of the
+// three- and fewer-particle entries (k12).
 struct QueueLds {
   double *hij;      // 6 * qcap
   double *coef;     // NS * qcap
@@ -1486,11 +1489,12 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
   //      order (k12);
   //   2. an exclusive scan of the counts gives each owner a range of rec,
   //      and the records are scattered into it;
-  //   3. each thread sorts its records by (entry, coupling) -- the values
-  //      are distinct, so the order is fixed -- and walks the k12 list:
-  //      before entry k it adds its records of earlier entries, then runs
-  //      get_d1 / get_d0 of entry k on its cells; the records after the
-  //      last such entry follow.
+  //   3. each thread sorts its records by (entry, slot) -- the values are
+  //      distinct, so the order is fixed -- and walks the k12 list: before
+  //      entry k it adds its records of earlier entries, then runs get_d1 /
+  //      get_d0 of entry k on its cells; entry k's own records (its folded
+  //      singles, on cells its lines never touch) and those of later entries
+  //      follow.
   auto cell_of = [&](int e, int c) -> uint32_t {
     return (Q.cellw[3 * e + (c >> 1)] >> (16 * (c & 1))) & 0xffffu;
   };
