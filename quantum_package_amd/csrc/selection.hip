@@ -1507,8 +1507,15 @@ __global__ __launch_bounds__(SB) void select_block_kernel(SelArgs A, Tabs T) {
     // entry, in the slots after its determinant; when they do not fit there
     // the replay computes them (flag bit 12).  kind_n = kind | records << 4
     // | first record slot << 8.
-    for (int e = tid; e < nq; e += SB) {
-      const int kind = Q.kind_n[e];
+    // Four-particle entries go to the lower half of the workgroup, the others
+    // to the upper half, so that the singles' integral loads overlap those
+    // of get_d2 instead of following them in the same lanes.
+    const int dhalf = tid / (SB / 2), dt = tid - dhalf * (SB / 2);
+    for (int e = dt; e < nq; e += SB / 2) {
+      // the other half may already have rewritten this entry's kind_n; the
+      // kind nibble stays
+      const int kind = Q.kind_n[e] & 0xf;
+      if ((kind == 0) != (dhalf == 0)) continue;
       const int idx = Q.idx[e];
       for (int st = 0; st < NS; ++st) Q.coef[(size_t)NS * e + st] = A.coef[(idx - 1) + A.n_det * st];
       C.gath += NS;
