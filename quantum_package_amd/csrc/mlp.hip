@@ -14,11 +14,12 @@
 //
 // GEMM: C(MxN) = A(MxK) B(KxN), A and B bf16 with arbitrary row/column
 // strides (so A^T, B^T need no copies), C fp32; workgroup tile 64x64, four
-// wavefronts each owning 32x32 = 2x2 tiles of v_mfma_f32_16x16x16_bf16.
-// Lane layout of the 16x16x16 MFMA: lane l holds A[l%16][4(l/16)+0..3],
-// B[4(l/16)+0..3][l%16] and D[4(l/16)+0..3][l%16].  Both tiles sit in LDS
+// wavefronts each owning 32x32 = 2x2 tiles of v_mfma_f32_16x16x32_bf16
+// (CDNA4's bf16 MFMA with K = 32, one per tile and K-tile of 32).
+// Lane layout of the 16x16x32 MFMA: lane l holds A[l%16][8(l/16)+0..7],
+// B[8(l/16)+0..7][l%16] and D[4(l/16)+0..3][l%16].  Both tiles sit in LDS
 // with k contiguous (A as [i][k], B as [j][k]), so each lane's MFMA operand
-// is one 8-byte read; they are staged in runs of 8 elements (16-byte loads)
+// is one 16-byte read; they are staged in runs of 8 elements (16-byte loads)
 // along whichever index of the operand is contiguous in memory.
 #include <algorithm>
 #include <cmath>
@@ -33,7 +34,7 @@ using namespace qpx;
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
@@ -121,7 +122,10 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
   // whole multiples of TK
   const int kc = ((g.K + g.ksplit - 1) / g.ksplit + TK - 1) / TK * TK;
   const int k_lo = (int)blockIdx.z * kc, k_hi = min(g.K, k_lo + kc);
-  const int r = lane & 15, q = (lane >> 4) * 4;
+  // A and B operands of v_mfma_f32_16x16x32_bf16: lane (r, g = lane / 16)
+  // supplies row r of the tile at k = 8 g .. 8 g + 7; the result rows are
+  // 4 g .. 4 g + 3 of column r
+  const int r = lane & 15, q = (lane >> 4) * 4, qk = (lane >> 4) * 8;
   for (int k0 = k_lo; k0 < k_hi; k0 += TK) {
     __syncthreads();
     // B(k, j) = B[k*b_rs + j*b_cs] is staged as the tile X(j, k)
@@ -129,17 +133,17 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
     stage_tile(Bs, g.B, g.b_cs, g.b_rs, g.b_run, j0, k0, g.N, k_hi, tid);
     __syncthreads();
 #pragma unroll
-    for (int ks = 0; ks < TK; ks += 16) {
-      bf16x4 av[2], bv[2];
+    for (int ks = 0; ks < TK; ks += 32) {
+      bf16x8 av[2], bv[2];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) av[a] = *(const bf16x4 *)&As[wi + 16 * a + r][ks + q];
+      for (int a = 0; a < 2; ++a) av[a] = *(const bf16x8 *)&As[wi + 16 * a + r][ks + qk];
 #pragma unroll
-      for (int b = 0; b < 2; ++b) bv[b] = *(const bf16x4 *)&Bs[wj + 16 * b + r][ks + q];
+      for (int b = 0; b < 2; ++b) bv[b] = *(const bf16x8 *)&Bs[wj + 16 * b + r][ks + qk];
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[a], bv[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
     }
   }
   // epilogue: lane holds D[q+0..3][r] of each 16x16 tile
